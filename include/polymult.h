@@ -1,0 +1,96 @@
+/* polymult.h -- C ABI of the MI355X negacyclic multiplier.
+ *
+ *   c = a * b mod (X^N + 1, q),  a, b, c: N coefficients of L little-endian
+ *   u32 limbs (row-major [N][L]), q a W-bit modulus of L limbs.
+ *
+ * Drop-in boundary. The reference binds three C++ entry points
+ * (fherma.h:46-48, bound by name in fherma.toml:7-10):
+ *   void* fherma_init(const fherma::Point&)            -> pm_init
+ *   fherma::Outputs fherma_run(void*, const Inputs&)   -> pm_run
+ *   void fherma_free(void*)                            -> pm_free
+ * Those three keep their C++ signatures in fherma.h (this directory) and are
+ * thin wrappers over the functions below, which take plain pointers and sizes
+ * so that any FFI (ctypes, cgo, JNI) can bind them.
+ *
+ * Errors: every int-returning call returns 0 on success and -1 on failure;
+ * pm_last_error() then returns the message (per thread), worded as the
+ * reference's exceptions (solve.cu:873-913, 1482-1483).
+ * Threading: one state is used by one thread at a time (the reference's
+ * state is not re-entrant either); distinct states are independent.
+ */
+#ifndef POLYMULT_H
+#define POLYMULT_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct pm_state pm_state;
+
+typedef struct pm_info {
+    uint32_t N, W, L;
+    uint32_t K;        /* RNS primes */
+    uint64_t delta;    /* 2^W - q */
+    uint32_t w1, w2;   /* NTT split: 2^w1 rows x 2^w2 columns */
+    uint64_t device_bytes;  /* device memory the state holds */
+} pm_info;
+
+/* Validates the point, builds the tables on the device, warms up.
+ * Replaces fherma_init (fherma.h:46; solve.cu:1392-1478). q has q_limbs
+ * limbs, which must equal L. */
+int pm_init(uint32_t N, uint32_t W, uint32_t L, const uint32_t* q, size_t q_limbs, pm_state** out);
+
+/* c = a * b from and to host memory (pageable or pinned), N*L u32 each,
+ * blocking. Replaces fherma_run (fherma.h:47; solve.cu:1480-1529). */
+int pm_run(pm_state* s, const uint32_t* a, const uint32_t* b, uint32_t* c);
+
+/* c = a * b with a, b, c already in device memory (N*L u32 each), enqueued
+ * on `stream` (a hipStream_t; NULL = the state's own stream). Returns once
+ * enqueued. No counterpart in the reference: this is the device-resident
+ * form of the same call. */
+int pm_run_device(pm_state* s, const uint32_t* d_a, const uint32_t* d_b, uint32_t* d_c, void* stream);
+
+/* Releases everything. Replaces fherma_free (fherma.h:48; solve.cu:1531-1535). */
+void pm_free(pm_state* s);
+
+const char* pm_last_error(void);
+int pm_query(const pm_state* s, pm_info* out);
+
+/* Host-only point check (no GPU needed): validates like pm_init and reports
+ * K, delta and up to `cap` primes. */
+int pm_point_info(uint32_t N, uint32_t W, uint32_t L, const uint32_t* q, size_t q_limbs, uint32_t* K,
+                  uint64_t* delta, uint32_t* primes, size_t cap);
+
+/* ---- device buffers and timing, for callers without another GPU runtime ---- */
+int pm_dev_alloc(size_t bytes, void** out);
+int pm_dev_free(void* p);
+int pm_copy_to_device(void* dst, const void* src, size_t bytes);
+int pm_copy_to_host(void* dst, const void* src, size_t bytes);
+int pm_sync(pm_state* s);
+
+/* Runs `iters` device-resident multiplies back to back on the state's stream
+ * and reports, from HIP events on that stream, the average milliseconds of
+ * the whole multiply and of each kernel: ms[0] whole, ms[1] RNS, ms[2]
+ * column forward, ms[3] fused rows, ms[4] column inverse, ms[5] CRT. */
+int pm_time_device(pm_state* s, const uint32_t* d_a, const uint32_t* d_b, uint32_t* d_c, int iters, float* ms);
+
+/* ---- stage entry points, for stage-by-stage parity tests ----
+ * All on device buffers, synchronous.
+ *   rns:     R[K][N] = a mod p_j
+ *   ntt_fwd: R <- forward negacyclic NTT of each prime row (in place),
+ *            bit-reversed output order
+ *   ntt_inv: R <- N * inverse NTT (unscaled)
+ *   crt:     c[N][L] from canonical residues X[K][N] of the integer product */
+int pm_stage_rns(pm_state* s, const uint32_t* d_a, uint32_t* d_R);
+int pm_stage_ntt_fwd(pm_state* s, uint32_t* d_R);
+int pm_stage_ntt_inv(pm_state* s, uint32_t* d_R);
+int pm_stage_crt(pm_state* s, const uint32_t* d_X, uint32_t* d_c);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif

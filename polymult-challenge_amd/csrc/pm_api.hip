@@ -1,0 +1,299 @@
+// Host runtime and C ABI: plan upload, the device-resident multiply, the
+// host-buffer multiply (chunked PCIe pipeline in pm_pipeline.cpp), stage
+// entry points and device-buffer helpers.
+#include <hip/hip_runtime.h>
+
+#include <cstring>
+#include <memory>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+#include "../../include/polymult.h"
+#include "pm_kernels.h"
+#include "pm_plan.h"
+#include "pm_state.h"
+
+namespace {
+
+thread_local std::string g_error;
+
+void hip_check(hipError_t e, const char* what) {
+    if (e != hipSuccess) throw std::runtime_error(std::string(what) + ": " + hipGetErrorString(e));
+}
+
+template <class F>
+int guarded(F&& body) {
+    try {
+        body();
+        return 0;
+    } catch (const std::exception& e) {
+        g_error = e.what();
+    } catch (...) {
+        g_error = "unknown error";
+    }
+    return -1;
+}
+
+// One device allocation for every constant table; returns the offset of each.
+struct TableUploader {
+    std::vector<unsigned char> blob;
+    size_t add(const void* data, size_t bytes) {
+        const size_t off = (blob.size() + 255) & ~size_t(255);
+        blob.resize(off + bytes);
+        if (bytes) std::memcpy(blob.data() + off, data, bytes);
+        return off;
+    }
+};
+
+}  // namespace
+
+namespace pm {
+
+void State::upload_plan() {
+    const HostPlan& H = host;
+    TableUploader U;
+    const size_t o_p = U.add(H.p.data(), H.p.size() * 4);
+    const size_t o_pinv = U.add(H.pinv.data(), H.pinv.size() * 4);
+    const size_t o_tw = U.add(H.tw.data(), H.tw.size() * 4);
+    const size_t o_rf = U.add(H.rns_frag.data(), H.rns_frag.size());
+    const size_t o_rb = U.add(H.rns_bias.data(), H.rns_bias.size() * 8);
+    const size_t o_cf = U.add(H.crt_frag.data(), H.crt_frag.size());
+    const size_t o_G = U.add(H.G.data(), H.G.size() * 4);
+    const size_t o_Gp = U.add(H.Gplain.data(), H.Gplain.size() * 4);
+    const size_t o_C0 = U.add(H.C0.data(), H.C0.size() * 4);
+    const size_t o_T = U.add(H.T.data(), H.T.size() * 4);
+    const size_t o_q = U.add(H.q.data(), H.q.size() * 4);
+    hip_check(hipMalloc(&d_tables, U.blob.size()), "hipMalloc tables");
+    hip_check(hipMemcpy(d_tables, U.blob.data(), U.blob.size(), hipMemcpyHostToDevice), "upload tables");
+    table_bytes = U.blob.size();
+    auto at = [&](size_t off) { return static_cast<unsigned char*>(d_tables) + off; };
+
+    DevPlan& D = dev;
+    D.N = H.N;
+    D.logN = H.logN;
+    D.w1 = H.w1;
+    D.w2 = H.w2;
+    D.K = H.K;
+    D.L = (int)H.L;
+    D.W = (int)H.W;
+    D.delta = H.delta;
+    D.p = reinterpret_cast<const uint32_t*>(at(o_p));
+    D.pinv = reinterpret_cast<const uint32_t*>(at(o_pinv));
+    D.tw = reinterpret_cast<const uint2*>(at(o_tw));
+    D.rns_frag = reinterpret_cast<const v4i*>(at(o_rf));
+    D.rns_bias = reinterpret_cast<const uint64_t*>(at(o_rb));
+    D.rns_rt = H.rns_rt;
+    D.rns_ks = H.rns_ks;
+    D.crt_frag = reinterpret_cast<const v4i*>(at(o_cf));
+    D.crt_rt = H.crt_rt;
+    D.crt_ks = H.crt_ks;
+    D.G = reinterpret_cast<const uint2*>(at(o_G));
+    D.Gplain = reinterpret_cast<const uint2*>(at(o_Gp));
+    D.C0 = reinterpret_cast<const uint32_t*>(at(o_C0));
+    D.F0 = H.F0;
+    D.T = reinterpret_cast<const uint32_t*>(at(o_T));
+    D.q = reinterpret_cast<const uint32_t*>(at(o_q));
+}
+
+void State::init(uint32_t N, uint32_t W, uint32_t L, const uint32_t* q, size_t q_limbs) {
+    plan_point(host, N, W, L, q, q_limbs);
+    build_tables(host);
+    hip_check(hipGetDevice(&device), "hipGetDevice");
+    int max_lds = 0;
+    hip_check(hipDeviceGetAttribute(&max_lds, hipDeviceAttributeMaxSharedMemoryPerBlock, device), "device attribute");
+    hip_check(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking), "stream");
+    upload_plan();
+    cfg = choose_launch(dev, std::min<size_t>((size_t)max_lds, 64 * 1024));
+    if (cfg.crt_lds > (size_t)max_lds || cfg.rns_lds > (size_t)max_lds)
+        throw std::runtime_error("point too large for this device's LDS");
+    coeff_words = (size_t)N * L;
+    const size_t rbytes = (size_t)host.K * N * 4;
+    for (int i = 0; i < 2; ++i) {
+        hip_check(hipMalloc(&d_R[i], rbytes), "hipMalloc residues");
+        hip_check(hipMalloc(&d_in[i], coeff_words * 4), "hipMalloc operands");
+    }
+    hip_check(hipMalloc(&d_out, coeff_words * 4), "hipMalloc result");
+    device_bytes = table_bytes + 2 * rbytes + 3 * coeff_words * 4;
+    pipeline_init();
+    // Warm-up on zero operands: loads every code object, touches every buffer.
+    hip_check(hipMemsetAsync(d_in[0], 0, coeff_words * 4, stream), "memset");
+    hip_check(hipMemsetAsync(d_in[1], 0, coeff_words * 4, stream), "memset");
+    hip_check(enqueue_multiply(d_in[0], d_in[1], d_out, stream), "warm-up multiply");
+    hip_check(hipStreamSynchronize(stream), "warm-up");
+}
+
+hipError_t State::enqueue_multiply(const uint32_t* a, const uint32_t* b, uint32_t* c, hipStream_t st) {
+    hipError_t e;
+    if ((e = launch_rns(dev, cfg, a, b, d_R[0], d_R[1], 2, st)) != hipSuccess) return e;
+    if ((e = launch_col(dev, cfg, d_R[0], d_R[1], 2, false, st)) != hipSuccess) return e;
+    if ((e = launch_row(dev, cfg, d_R[1], d_R[0], ROW_FUSED, st)) != hipSuccess) return e;
+    if ((e = launch_col(dev, cfg, d_R[1], nullptr, 1, true, st)) != hipSuccess) return e;
+    return launch_crt(dev, cfg, d_R[1], c, dev.G, st);
+}
+
+State::~State() {
+    if (stream) (void)hipStreamSynchronize(stream);
+    pipeline_free();
+    for (int i = 0; i < 2; ++i) {
+        if (d_R[i]) (void)hipFree(d_R[i]);
+        if (d_in[i]) (void)hipFree(d_in[i]);
+    }
+    if (d_out) (void)hipFree(d_out);
+    if (d_tables) (void)hipFree(d_tables);
+    if (stream) (void)hipStreamDestroy(stream);
+}
+
+}  // namespace pm
+
+struct pm_state {
+    std::unique_ptr<pm::State> s;
+};
+
+extern "C" {
+
+const char* pm_last_error(void) { return g_error.c_str(); }
+
+int pm_point_info(uint32_t N, uint32_t W, uint32_t L, const uint32_t* q, size_t q_limbs, uint32_t* K,
+                  uint64_t* delta, uint32_t* primes, size_t cap) {
+    return guarded([&] {
+        pm::HostPlan P;
+        pm::plan_point(P, N, W, L, q, q_limbs);
+        if (K) *K = (uint32_t)P.K;
+        if (delta) *delta = P.delta;
+        if (primes)
+            for (size_t i = 0; i < cap && i < P.p.size(); ++i) primes[i] = P.p[i];
+    });
+}
+
+int pm_init(uint32_t N, uint32_t W, uint32_t L, const uint32_t* q, size_t q_limbs, pm_state** out) {
+    return guarded([&] {
+        if (!out) throw std::runtime_error("null output handle");
+        *out = nullptr;
+        auto h = std::make_unique<pm_state>();
+        h->s = std::make_unique<pm::State>();
+        h->s->init(N, W, L, q, q_limbs);
+        *out = h.release();
+    });
+}
+
+void pm_free(pm_state* s) { delete s; }
+
+int pm_query(const pm_state* s, pm_info* out) {
+    return guarded([&] {
+        if (!s || !out) throw std::runtime_error("null argument");
+        const pm::HostPlan& H = s->s->host;
+        out->N = H.N;
+        out->W = H.W;
+        out->L = H.L;
+        out->K = (uint32_t)H.K;
+        out->delta = H.delta;
+        out->w1 = (uint32_t)H.w1;
+        out->w2 = (uint32_t)H.w2;
+        out->device_bytes = s->s->device_bytes;
+    });
+}
+
+int pm_run(pm_state* s, const uint32_t* a, const uint32_t* b, uint32_t* c) {
+    return guarded([&] {
+        if (!s || !a || !b || !c) throw std::runtime_error("null argument");
+        s->s->run_host(a, b, c);
+    });
+}
+
+int pm_run_device(pm_state* s, const uint32_t* d_a, const uint32_t* d_b, uint32_t* d_c, void* stream) {
+    return guarded([&] {
+        if (!s || !d_a || !d_b || !d_c) throw std::runtime_error("null argument");
+        hipStream_t st = stream ? static_cast<hipStream_t>(stream) : s->s->stream;
+        hip_check(s->s->enqueue_multiply(d_a, d_b, d_c, st), "multiply");
+    });
+}
+
+int pm_sync(pm_state* s) {
+    return guarded([&] { hip_check(hipStreamSynchronize(s->s->stream), "synchronize"); });
+}
+
+int pm_dev_alloc(size_t bytes, void** out) {
+    return guarded([&] { hip_check(hipMalloc(out, bytes ? bytes : 1), "hipMalloc"); });
+}
+int pm_dev_free(void* p) {
+    return guarded([&] { hip_check(hipFree(p), "hipFree"); });
+}
+int pm_copy_to_device(void* dst, const void* src, size_t bytes) {
+    return guarded([&] { hip_check(hipMemcpy(dst, src, bytes, hipMemcpyHostToDevice), "copy to device"); });
+}
+int pm_copy_to_host(void* dst, const void* src, size_t bytes) {
+    return guarded([&] { hip_check(hipMemcpy(dst, src, bytes, hipMemcpyDeviceToHost), "copy to host"); });
+}
+
+int pm_time_device(pm_state* h, const uint32_t* d_a, const uint32_t* d_b, uint32_t* d_c, int iters, float* ms) {
+    return guarded([&] {
+        pm::State& S = *h->s;
+        const pm::DevPlan& D = S.dev;
+        const pm::LaunchCfg& C = S.cfg;
+        hipStream_t st = S.stream;
+        constexpr int NE = 6;
+        hipEvent_t ev[NE];
+        for (auto& e : ev) hip_check(hipEventCreate(&e), "event");
+        double acc[NE] = {0};
+        for (int it = 0; it < iters; ++it) {
+            hip_check(hipEventRecord(ev[0], st), "record");
+            hip_check(pm::launch_rns(D, C, d_a, d_b, S.d_R[0], S.d_R[1], 2, st), "rns");
+            hip_check(hipEventRecord(ev[1], st), "record");
+            hip_check(pm::launch_col(D, C, S.d_R[0], S.d_R[1], 2, false, st), "col");
+            hip_check(hipEventRecord(ev[2], st), "record");
+            hip_check(pm::launch_row(D, C, S.d_R[1], S.d_R[0], pm::ROW_FUSED, st), "row");
+            hip_check(hipEventRecord(ev[3], st), "record");
+            hip_check(pm::launch_col(D, C, S.d_R[1], nullptr, 1, true, st), "col inv");
+            hip_check(hipEventRecord(ev[4], st), "record");
+            hip_check(pm::launch_crt(D, C, S.d_R[1], d_c, D.G, st), "crt");
+            hip_check(hipEventRecord(ev[5], st), "record");
+            hip_check(hipEventSynchronize(ev[5]), "sync");
+            float t = 0;
+            hip_check(hipEventElapsedTime(&t, ev[0], ev[5]), "elapsed");
+            acc[0] += t;
+            for (int k = 1; k < NE; ++k) {
+                hip_check(hipEventElapsedTime(&t, ev[k - 1], ev[k]), "elapsed");
+                acc[k] += t;
+            }
+        }
+        for (auto& e : ev) (void)hipEventDestroy(e);
+        for (int k = 0; k < NE; ++k) ms[k] = (float)(acc[k] / (iters > 0 ? iters : 1));
+    });
+}
+
+int pm_stage_rns(pm_state* h, const uint32_t* d_a, uint32_t* d_R) {
+    return guarded([&] {
+        pm::State& S = *h->s;
+        hip_check(pm::launch_rns(S.dev, S.cfg, d_a, nullptr, d_R, nullptr, 1, S.stream), "rns");
+        hip_check(hipStreamSynchronize(S.stream), "sync");
+    });
+}
+
+int pm_stage_ntt_fwd(pm_state* h, uint32_t* d_R) {
+    return guarded([&] {
+        pm::State& S = *h->s;
+        hip_check(pm::launch_col(S.dev, S.cfg, d_R, nullptr, 1, false, S.stream), "col");
+        hip_check(pm::launch_row(S.dev, S.cfg, d_R, nullptr, pm::ROW_FWD, S.stream), "row");
+        hip_check(hipStreamSynchronize(S.stream), "sync");
+    });
+}
+
+int pm_stage_ntt_inv(pm_state* h, uint32_t* d_R) {
+    return guarded([&] {
+        pm::State& S = *h->s;
+        hip_check(pm::launch_row(S.dev, S.cfg, d_R, nullptr, pm::ROW_INV, S.stream), "row");
+        hip_check(pm::launch_col(S.dev, S.cfg, d_R, nullptr, 1, true, S.stream), "col");
+        hip_check(hipStreamSynchronize(S.stream), "sync");
+    });
+}
+
+int pm_stage_crt(pm_state* h, const uint32_t* d_X, uint32_t* d_c) {
+    return guarded([&] {
+        pm::State& S = *h->s;
+        hip_check(pm::launch_crt(S.dev, S.cfg, d_X, d_c, S.dev.Gplain, S.stream), "crt");
+        hip_check(hipStreamSynchronize(S.stream), "sync");
+    });
+}
+
+}  // extern "C"

@@ -1,0 +1,64 @@
+// Device view of a plan and the kernel launchers (pm_kernels.hip).
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+
+namespace pm {
+
+typedef int v4i __attribute__((ext_vector_type(4)));
+
+// Everything a kernel needs, passed by value.
+struct DevPlan {
+    uint32_t N;
+    int logN, w1, w2, K, L, W;
+    uint64_t delta;
+    const uint32_t* p;     // [K]
+    const uint32_t* pinv;  // [K]
+    const uint2* tw;       // [K][N] Shoup pairs of psi^bitrev(k)
+    const v4i* rns_frag;   // [rns_rt][rns_ks][64]
+    const uint64_t* rns_bias;
+    int rns_rt, rns_ks;
+    const v4i* crt_frag;   // [crt_rt][crt_ks][64]
+    int crt_rt, crt_ks;
+    const uint2* G;        // [K]
+    const uint2* Gplain;   // [K]
+    const uint32_t* C0;    // [L + 2]
+    int64_t F0;
+    const uint32_t* T;     // [K + 1][L]
+    const uint32_t* q;     // [L]
+};
+
+// Launch geometry chosen once per plan.
+struct LaunchCfg {
+    int rns_tn = 128;   // coefficients per RNS block (32, 64 or 128)
+    int rns_stride = 0; // LDS bytes per coefficient row
+    size_t rns_lds = 0;
+    int crt_tn = 64;    // coefficients per CRT block
+    int crt_ys = 0;     // LDS bytes per coefficient of y bytes
+    size_t crt_lds = 0;
+    int col_tc = 32;    // columns per column-pass block
+    int row_rpb = 16;   // rows per row-pass block
+};
+
+enum RowMode { ROW_FUSED = 0, ROW_FWD = 1, ROW_INV = 2 };
+
+// Chooses block shapes for a plan under the device's LDS limit.
+LaunchCfg choose_launch(const DevPlan& D, size_t max_lds);
+
+// R_i[j][n] = a_i[n] mod p_j for nops (1 or 2) operands.
+hipError_t launch_rns(const DevPlan& D, const LaunchCfg& C, const uint32_t* a0, const uint32_t* a1,
+                      uint32_t* R0, uint32_t* R1, int nops, hipStream_t s);
+// Column pass (stages 0 .. w1-1, or their inverse) on nops residue arrays.
+hipError_t launch_col(const DevPlan& D, const LaunchCfg& C, uint32_t* R0, uint32_t* R1, int nops, bool inverse,
+                      hipStream_t s);
+// Row pass. ROW_FUSED: RB <- inv(fwd(RB) * fwd(RA)) per row (RA untouched,
+// the product carries 2^-32). ROW_FWD / ROW_INV: RB only.
+hipError_t launch_row(const DevPlan& D, const LaunchCfg& C, uint32_t* RB, const uint32_t* RA, RowMode mode,
+                      hipStream_t s);
+// out[n][L] = CRT of R[j][n] (scaled by G) mod q.
+hipError_t launch_crt(const DevPlan& D, const LaunchCfg& C, const uint32_t* R, uint32_t* out, const uint2* G,
+                      hipStream_t s);
+
+}  // namespace pm

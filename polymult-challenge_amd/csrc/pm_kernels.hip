@@ -1,0 +1,582 @@
+// gfx950 kernels of the negacyclic multiply c = a b mod (X^N + 1, q).
+//
+//   k_rns   a[N][L] -> R[K][N]: residues as an exact int8 MFMA GEMM over the
+//           bytes of each coefficient (v_mfma_i32_32x32x32_i8), one REDC.
+//   k_col   NTT stages 0 .. w1-1 (or their inverse) on column tiles.
+//   k_row   NTT stages w1 .. logN-1 on rows; the fused mode runs a's and b's
+//           forward rows, the pointwise product and b's inverse rows in one
+//           pass over the data.
+//   k_crt   R[K][N] -> c[N][L]: y_j = r_j G_j, then z = sum_j y_j Q_j and the
+//           estimate of v = round(sum_j y_j / p_j) as one int8 MFMA GEMM over
+//           the bytes of y, then + T[v], the 2^W = delta fold and one -q.
+//
+// Residues live as R[j][n] with n = row 2^w2 + col: one contiguous vector per
+// prime, rows contiguous, columns strided. Every NTT kernel keeps 16 values
+// per thread in registers and does four radix-2 stages there; the two halves
+// of an 8-stage pass meet through one LDS transpose.
+#include "pm_kernels.h"
+
+#include "pm_arith.h"
+
+namespace pm {
+namespace {
+
+typedef uint32_t u32;
+typedef uint64_t u64;
+typedef int v16i __attribute__((ext_vector_type(16)));
+
+__device__ __forceinline__ v16i mfma_i8(v4i a, v4i b, v16i c) {
+    return __builtin_amdgcn_mfma_i32_32x32x32_i8(a, b, c, 0, 0, 0);
+}
+
+// ---------------------------------------------------------------------------
+// NTT lines
+//
+// A line is 2^NB consecutive NTT positions of one pass: a column (stride
+// ncols) in the column pass, a row in the row pass. T threads share a line,
+// each holding E registers. Pass 1 owns positions t + k T and runs the first
+// log2(E) stages; pass 2 owns positions t E + k and runs the rest. Stage ls of
+// a line with global stage offset s0 and line index `line` uses twiddle
+// tw[2^(s0+ls) + line 2^ls + i], i the block index inside the line; the
+// inverse uses the mirrored entry tw[2^(s0+ls+1) - 1 - (line 2^ls + i)].
+// ---------------------------------------------------------------------------
+
+template <int NB>
+struct Line {
+    static constexpr int LE = NB >= 4 ? 4 : NB;
+    static constexpr int E = 1 << LE;
+    static constexpr int T = (1 << NB) / E;
+    static constexpr int R1 = LE;
+    static constexpr int R2 = NB - LE;
+};
+
+template <int NB, int NA>
+__device__ __forceinline__ void fwd_pass1(u32 (&x)[NA][Line<NB>::E], const uint2* __restrict__ tab, int s0,
+                                          u32 line, u32 p) {
+    constexpr int E = Line<NB>::E, R1 = Line<NB>::R1;
+#pragma unroll
+    for (int st = 0; st < R1; ++st) {
+        const int half = E >> (st + 1);
+#pragma unroll
+        for (int b = 0; b < (1 << st); ++b) {
+            const uint2 w = tab[(1u << (s0 + st)) + (line << st) + b];
+#pragma unroll
+            for (int a = 0; a < NA; ++a)
+#pragma unroll
+                for (int qq = 0; qq < half; ++qq)
+                    ct_butterfly(x[a][2 * b * half + qq], x[a][2 * b * half + qq + half], w.x, w.y, p);
+        }
+    }
+}
+
+template <int NB, int NA>
+__device__ __forceinline__ void fwd_pass2(u32 (&x)[NA][Line<NB>::E], const uint2* __restrict__ tab, int s0,
+                                          u32 line, int t, u32 p) {
+    constexpr int E = Line<NB>::E, R1 = Line<NB>::R1, R2 = Line<NB>::R2;
+#pragma unroll
+    for (int st = 0; st < R2; ++st) {
+        const int ls = R1 + st;
+        const int half = 1 << (R2 - 1 - st);
+        const int nblk = E / (2 * half);
+#pragma unroll
+        for (int b = 0; b < nblk; ++b) {
+            const uint2 w = tab[(1u << (s0 + ls)) + (line << ls) + (u32)(t * nblk + b)];
+#pragma unroll
+            for (int a = 0; a < NA; ++a)
+#pragma unroll
+                for (int qq = 0; qq < half; ++qq)
+                    ct_butterfly(x[a][2 * b * half + qq], x[a][2 * b * half + qq + half], w.x, w.y, p);
+        }
+    }
+}
+
+template <int NB>
+__device__ __forceinline__ void inv_pass2(u32 (&x)[Line<NB>::E], const uint2* __restrict__ tab, int s0, u32 line,
+                                          int t, u32 p) {
+    constexpr int E = Line<NB>::E, R1 = Line<NB>::R1, R2 = Line<NB>::R2;
+#pragma unroll
+    for (int st = R2 - 1; st >= 0; --st) {
+        const int ls = R1 + st;
+        const int half = 1 << (R2 - 1 - st);
+        const int nblk = E / (2 * half);
+#pragma unroll
+        for (int b = 0; b < nblk; ++b) {
+            const u32 i = (line << ls) + (u32)(t * nblk + b);
+            const uint2 m = tab[(2u << (s0 + ls)) - 1u - i];
+#pragma unroll
+            for (int qq = 0; qq < half; ++qq) gs_butterfly(x[2 * b * half + qq], x[2 * b * half + qq + half], m.x, m.y, p);
+        }
+    }
+}
+
+template <int NB>
+__device__ __forceinline__ void inv_pass1(u32 (&x)[Line<NB>::E], const uint2* __restrict__ tab, int s0, u32 line,
+                                          u32 p) {
+    constexpr int E = Line<NB>::E, R1 = Line<NB>::R1;
+#pragma unroll
+    for (int st = R1 - 1; st >= 0; --st) {
+        const int half = E >> (st + 1);
+#pragma unroll
+        for (int b = 0; b < (1 << st); ++b) {
+            const u32 i = (line << st) + (u32)b;
+            const uint2 m = tab[(2u << (s0 + st)) - 1u - i];
+#pragma unroll
+            for (int qq = 0; qq < half; ++qq) gs_butterfly(x[2 * b * half + qq], x[2 * b * half + qq + half], m.x, m.y, p);
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Column pass. Block = tc columns x T threads; LDS tile [2^NB][tc] for the
+// transpose between the two register passes. Threads sweep columns fastest,
+// so every global access of a wave reads or writes whole row segments.
+// ---------------------------------------------------------------------------
+template <int NB, bool INV>
+__global__ void __launch_bounds__(1024) k_col(DevPlan D, u32* __restrict__ R0, u32* __restrict__ R1, int tc) {
+    using Ln = Line<NB>;
+    constexpr int E = Ln::E, T = Ln::T, R2 = Ln::R2;
+    extern __shared__ u32 lds[];
+    const int c = (int)threadIdx.x % tc, t = (int)threadIdx.x / tc;
+    u32* R = blockIdx.z ? R1 : R0;
+    const int j = (int)blockIdx.y;
+    const u32 p = D.p[j];
+    const uint2* tab = D.tw + (size_t)j * D.N;
+    const size_t ncols = (size_t)1 << D.w2;
+    u32* col = R + (size_t)j * D.N + (size_t)blockIdx.x * tc + c;
+    u32 x[1][E];
+    if (!INV) {
+#pragma unroll
+        for (int k = 0; k < E; ++k) x[0][k] = col[(size_t)(t + k * T) * ncols];
+        fwd_pass1<NB, 1>(x, tab, 0, 0, p);
+        if constexpr (R2 > 0) {
+#pragma unroll
+            for (int k = 0; k < E; ++k) lds[(t + k * T) * tc + c] = x[0][k];
+            __syncthreads();
+#pragma unroll
+            for (int k = 0; k < E; ++k) x[0][k] = lds[(t * E + k) * tc + c];
+            fwd_pass2<NB, 1>(x, tab, 0, 0, t, p);
+#pragma unroll
+            for (int k = 0; k < E; ++k) col[(size_t)(t * E + k) * ncols] = x[0][k];
+        } else {
+#pragma unroll
+            for (int k = 0; k < E; ++k) col[(size_t)(t + k * T) * ncols] = x[0][k];
+        }
+    } else {
+        if constexpr (R2 > 0) {
+#pragma unroll
+            for (int k = 0; k < E; ++k) x[0][k] = col[(size_t)(t * E + k) * ncols];
+            inv_pass2<NB>(x[0], tab, 0, 0, t, p);
+#pragma unroll
+            for (int k = 0; k < E; ++k) lds[(t * E + k) * tc + c] = x[0][k];
+            __syncthreads();
+#pragma unroll
+            for (int k = 0; k < E; ++k) x[0][k] = lds[(t + k * T) * tc + c];
+        } else {
+#pragma unroll
+            for (int k = 0; k < E; ++k) x[0][k] = col[(size_t)(t + k * T) * ncols];
+        }
+        inv_pass1<NB>(x[0], tab, 0, 0, p);
+#pragma unroll
+        for (int k = 0; k < E; ++k) col[(size_t)(t + k * T) * ncols] = x[0][k];
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Row pass. Block = rpb rows x T threads. LDS rows are padded by 16 words and
+// XOR-swizzled inside each 16-word group so that both the strided pass-1
+// writes and the contiguous pass-2 reads spread over the banks.
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ int swz(int e) { return e ^ ((e >> 4) & 15); }
+
+template <int NB, int MODE>
+__global__ void __launch_bounds__(256) k_row(DevPlan D, u32* __restrict__ RB, const u32* __restrict__ RA, int rpb) {
+    using Ln = Line<NB>;
+    constexpr int E = Ln::E, T = Ln::T, R2 = Ln::R2;
+    constexpr int RS = (1 << NB) + (R2 > 0 ? 16 : 0);
+    extern __shared__ u32 lds[];
+    const int r = (int)threadIdx.x / T, t = (int)threadIdx.x % T;
+    const int j = (int)blockIdx.y;
+    const u32 row = blockIdx.x * (u32)rpb + (u32)r;
+    const u32 p = D.p[j];
+    const uint2* tab = D.tw + (size_t)j * D.N;
+    const size_t off = (size_t)j * D.N + ((size_t)row << NB);
+    u32* rb = RB + off;
+    const int s0 = D.w1;
+    u32* lb = lds + r * RS;
+    u32* la = lds + (rpb + r) * RS;
+
+    if constexpr (MODE == ROW_FUSED) {
+        const u32* ra = RA + off;
+        const u32 pinv = D.pinv[j];
+        u32 x[2][E];
+#pragma unroll
+        for (int k = 0; k < E; ++k) {
+            x[0][k] = ra[t + k * T];
+            x[1][k] = rb[t + k * T];
+        }
+        fwd_pass1<NB, 2>(x, tab, s0, row, p);
+        if constexpr (R2 > 0) {
+#pragma unroll
+            for (int k = 0; k < E; ++k) {
+                la[swz(t + k * T)] = x[0][k];
+                lb[swz(t + k * T)] = x[1][k];
+            }
+            __syncthreads();
+#pragma unroll
+            for (int k = 0; k < E; ++k) {
+                x[0][k] = la[swz(t * E + k)];
+                x[1][k] = lb[swz(t * E + k)];
+            }
+            fwd_pass2<NB, 2>(x, tab, s0, row, t, p);
+        }
+        u32 y[E];
+#pragma unroll
+        for (int k = 0; k < E; ++k) y[k] = mont_mul(x[0][k], x[1][k], p, pinv);
+        if constexpr (R2 > 0) {
+            inv_pass2<NB>(y, tab, s0, row, t, p);
+            __syncthreads();
+#pragma unroll
+            for (int k = 0; k < E; ++k) lb[swz(t * E + k)] = y[k];
+            __syncthreads();
+#pragma unroll
+            for (int k = 0; k < E; ++k) y[k] = lb[swz(t + k * T)];
+        }
+        inv_pass1<NB>(y, tab, s0, row, p);
+#pragma unroll
+        for (int k = 0; k < E; ++k) rb[t + k * T] = y[k];
+    } else if constexpr (MODE == ROW_FWD) {
+        u32 x[1][E];
+#pragma unroll
+        for (int k = 0; k < E; ++k) x[0][k] = rb[t + k * T];
+        fwd_pass1<NB, 1>(x, tab, s0, row, p);
+        if constexpr (R2 > 0) {
+#pragma unroll
+            for (int k = 0; k < E; ++k) lb[swz(t + k * T)] = x[0][k];
+            __syncthreads();
+#pragma unroll
+            for (int k = 0; k < E; ++k) x[0][k] = lb[swz(t * E + k)];
+            fwd_pass2<NB, 1>(x, tab, s0, row, t, p);
+#pragma unroll
+            for (int k = 0; k < E; ++k) rb[t * E + k] = x[0][k];
+        } else {
+#pragma unroll
+            for (int k = 0; k < E; ++k) rb[t + k * T] = x[0][k];
+        }
+    } else {  // ROW_INV
+        u32 y[E];
+        if constexpr (R2 > 0) {
+#pragma unroll
+            for (int k = 0; k < E; ++k) y[k] = rb[t * E + k];
+            inv_pass2<NB>(y, tab, s0, row, t, p);
+#pragma unroll
+            for (int k = 0; k < E; ++k) lb[swz(t * E + k)] = y[k];
+            __syncthreads();
+#pragma unroll
+            for (int k = 0; k < E; ++k) y[k] = lb[swz(t + k * T)];
+        } else {
+#pragma unroll
+            for (int k = 0; k < E; ++k) y[k] = rb[t + k * T];
+        }
+        inv_pass1<NB>(y, tab, s0, row, p);
+#pragma unroll
+        for (int k = 0; k < E; ++k) rb[t + k * T] = y[k];
+    }
+}
+
+// ---------------------------------------------------------------------------
+// RNS decomposition. Block = 4 waves and 32 NCT coefficients, staged in LDS
+// as bytes (row stride `stride`). The GEMM is C[(j,t)][n] = sum_b D[(j,t)][b]
+// (byte_b(a_n) - 128), D the balanced digits of 2^(8b+32) mod p_j; then
+// sum_t 256^t C + bias_j = a_n 2^32 (mod p_j) is positive and below p 2^32,
+// and one REDC gives a_n mod p_j.
+//
+// MFMA 32x32x32 lane layout used here: A lane l holds row l%32, k = 16(l/32)
+// + 0..15; B lane l holds column l%32 and the same k; the result register r
+// of lane l is row 8(r/4) + 4(l/32) + r%4, column l%32.
+// ---------------------------------------------------------------------------
+template <int NCT>
+__global__ void __launch_bounds__(256) k_rns(DevPlan D, const u32* __restrict__ a0, const u32* __restrict__ a1,
+                                              u32* __restrict__ R0, u32* __restrict__ R1, int stride) {
+    extern __shared__ __align__(16) unsigned char lds8[];
+    constexpr int TN = 32 * NCT;
+    const u32* a = blockIdx.y ? a1 : a0;
+    u32* R = blockIdx.y ? R1 : R0;
+    const int L = D.L, K = D.K;
+    const u32 N = D.N;
+    const u32 n0 = blockIdx.x * (u32)TN;
+    const int cnt = (int)min((u32)TN, N - n0);
+    const u32* src = a + (size_t)n0 * L;
+    const int kb = 32 * D.rns_ks;  // padded bytes per coefficient
+
+    for (int idx = (int)threadIdx.x; idx < TN * L; idx += (int)blockDim.x) {
+        const int c = idx / L, l = idx - c * L;
+        const u32 v = c < cnt ? src[idx] : 0u;
+        *reinterpret_cast<u32*>(lds8 + c * stride + 4 * l) = v;
+    }
+    const int padw = (kb - 4 * L) / 4;
+    for (int idx = (int)threadIdx.x; idx < TN * padw; idx += (int)blockDim.x) {
+        const int c = idx / padw, w = idx - c * padw;
+        *reinterpret_cast<u32*>(lds8 + c * stride + 4 * L + 4 * w) = 0x80808080u;
+    }
+    __syncthreads();
+
+    const int lane = (int)threadIdx.x & 63, wave = (int)threadIdx.x >> 6, nw = (int)blockDim.x >> 6;
+    const int half = lane >> 5, col = lane & 31;
+    const v4i flip = {(int)0x80808080u, (int)0x80808080u, (int)0x80808080u, (int)0x80808080u};
+    for (int rt = wave; rt < D.rns_rt; rt += nw) {
+        v16i acc[NCT];
+#pragma unroll
+        for (int ct = 0; ct < NCT; ++ct) acc[ct] = (v16i){0};
+        const v4i* frag = D.rns_frag + (size_t)rt * D.rns_ks * 64 + lane;
+        for (int ks = 0; ks < D.rns_ks; ++ks) {
+            const v4i A = frag[(size_t)ks * 64];
+#pragma unroll
+            for (int ct = 0; ct < NCT; ++ct) {
+                const v4i B = *reinterpret_cast<const v4i*>(lds8 + (32 * ct + col) * stride + 32 * ks + 16 * half) ^ flip;
+                acc[ct] = mfma_i8(A, B, acc[ct]);
+            }
+        }
+#pragma unroll
+        for (int g = 0; g < 4; ++g) {
+            const int j = 8 * rt + 2 * g + half;
+            if (j >= K) continue;
+            const u32 p = D.p[j], pinv = D.pinv[j];
+            const long long bias = (long long)D.rns_bias[j];
+#pragma unroll
+            for (int ct = 0; ct < NCT; ++ct) {
+                const int c = 32 * ct + col;
+                if (c >= cnt) continue;
+                const long long s = (long long)acc[ct][4 * g] + (long long)acc[ct][4 * g + 1] * 256 +
+                                    (long long)acc[ct][4 * g + 2] * 65536 + (long long)acc[ct][4 * g + 3] * 16777216;
+                R[(size_t)j * N + n0 + c] = redc64((u64)(s + bias), p, pinv);
+            }
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------
+// CRT. Block = 4 waves and TN = 32 NCT coefficients.
+//   A: y_j = r_j G_j, stored as 4K bytes per coefficient, re-centred (^0x80).
+//   B: MFMA over (j, s) against the digit table: slot i of the result is
+//      sum_t 256^t C[4i + t]; slots 0..L give z - C0, slots L+1, L+2 give
+//      F - F0 with F = sum_j y_j floor(2^48 / p_j).
+//   C: one thread per coefficient: v = (F + 2^47) >> 48, carry-propagate
+//      z = sum_i P_i 2^(32 i) + C0 + T[v] into L + 2 limbs, fold with
+//      2^W = delta until z < 2^W, subtract q once if needed.
+//   D: coalesced store of the block's contiguous [TN][L] output.
+// ---------------------------------------------------------------------------
+template <int NCT>
+__global__ void __launch_bounds__(256) k_crt(DevPlan D, const u32* __restrict__ R, u32* __restrict__ out,
+                                              const uint2* __restrict__ G, int ys) {
+    extern __shared__ __align__(16) unsigned char lds8[];
+    constexpr int TN = 32 * NCT;
+    const int L = D.L, K = D.K, nslot = L + 3, ZL = L + 2;
+    const u32 N = D.N;
+    unsigned char* Y = lds8;                                           // [TN][ys]
+    long long* Pp = reinterpret_cast<long long*>(lds8 + (size_t)TN * ys);  // [TN][nslot]
+    const u32 n0 = blockIdx.x * (u32)TN;
+    const int cnt = (int)min((u32)TN, N - n0);
+
+    const int kp = 8 * D.crt_ks;
+    for (int idx = (int)threadIdx.x; idx < kp * TN; idx += (int)blockDim.x) {
+        const int j = idx / TN, c = idx - j * TN;
+        u32 y = 0;
+        if (j < K && c < cnt) {
+            const uint2 g = G[j];
+            y = shoup_mul(R[(size_t)j * N + n0 + c], g.x, g.y, D.p[j]);
+        }
+        *reinterpret_cast<u32*>(Y + c * ys + 4 * j) = y ^ 0x80808080u;
+    }
+    __syncthreads();
+
+    const int lane = (int)threadIdx.x & 63, wave = (int)threadIdx.x >> 6, nw = (int)blockDim.x >> 6;
+    const int half = lane >> 5, col = lane & 31;
+    for (int rt = wave; rt < D.crt_rt; rt += nw) {
+        v16i acc[NCT];
+#pragma unroll
+        for (int ct = 0; ct < NCT; ++ct) acc[ct] = (v16i){0};
+        const v4i* frag = D.crt_frag + (size_t)rt * D.crt_ks * 64 + lane;
+        for (int ks = 0; ks < D.crt_ks; ++ks) {
+            const v4i A = frag[(size_t)ks * 64];
+#pragma unroll
+            for (int ct = 0; ct < NCT; ++ct) {
+                const v4i B = *reinterpret_cast<const v4i*>(Y + (32 * ct + col) * ys + 32 * ks + 16 * half);
+                acc[ct] = mfma_i8(A, B, acc[ct]);
+            }
+        }
+#pragma unroll
+        for (int g = 0; g < 4; ++g) {
+            const int slot = 8 * rt + 2 * g + half;
+            if (slot >= nslot) continue;
+#pragma unroll
+            for (int ct = 0; ct < NCT; ++ct) {
+                const long long s = (long long)acc[ct][4 * g] + (long long)acc[ct][4 * g + 1] * 256 +
+                                    (long long)acc[ct][4 * g + 2] * 65536 + (long long)acc[ct][4 * g + 3] * 16777216;
+                Pp[(size_t)(32 * ct + col) * nslot + slot] = s;
+            }
+        }
+    }
+    __syncthreads();
+
+    u32* Z = reinterpret_cast<u32*>(Y);  // [TN][ZL], reuses the y bytes
+    const int W = D.W, wl = W >> 5, wb = W & 31;
+    for (int c = (int)threadIdx.x; c < cnt; c += (int)blockDim.x) {
+        const long long* P = Pp + (size_t)c * nslot;
+        u32* z = Z + (size_t)c * ZL;
+        const u64 F = (u64)P[L + 1] + ((u64)P[L + 2] << 32) + (u64)D.F0;
+        // v <= K for every valid input; the clamp only keeps a corrupted sum
+        // from indexing past T.
+        const u32 v = min((u32)((F + (1ull << 47)) >> 48), (u32)K);
+        const u32* Tv = D.T + (size_t)v * L;
+        long long carry = 0;
+        for (int i = 0; i < ZL; ++i) {
+            long long s = carry + (long long)D.C0[i];
+            if (i <= L) s += P[i];
+            if (i < L) s += Tv[i];
+            z[i] = (u32)s;
+            carry = s >> 32;  // arithmetic: the running sum may be negative
+        }
+        for (;;) {
+            u64 hi;
+            if (wb == 0) {
+                hi = (u64)z[wl] | ((u64)z[wl + 1] << 32);
+            } else {
+                hi = ((u64)z[wl] >> wb) | ((u64)z[wl + 1] << (32 - wb));
+                if (wl + 2 < ZL) hi |= (u64)z[wl + 2] << (64 - wb);
+            }
+            if (hi == 0) break;
+            z[wl] = wb ? (z[wl] & ((1u << wb) - 1u)) : 0u;
+            for (int i = wl + 1; i < ZL; ++i) z[i] = 0;
+            const u64 lo64 = hi * D.delta, hi64 = __umul64hi(hi, D.delta);
+            const u32 add[4] = {(u32)lo64, (u32)(lo64 >> 32), (u32)hi64, (u32)(hi64 >> 32)};
+            u64 cy = 0;
+            for (int i = 0; i < ZL; ++i) {
+                const u64 s = (u64)z[i] + (i < 4 ? add[i] : 0u) + cy;
+                z[i] = (u32)s;
+                cy = s >> 32;
+                if (i >= 3 && cy == 0) break;
+            }
+        }
+        bool ge = true;
+        for (int i = L - 1; i >= 0; --i) {
+            const u32 qi = D.q[i];
+            if (z[i] != qi) {
+                ge = z[i] > qi;
+                break;
+            }
+        }
+        if (ge) {
+            long long br = 0;
+            for (int i = 0; i < L; ++i) {
+                const long long d = (long long)z[i] - D.q[i] - br;
+                z[i] = (u32)d;
+                br = d < 0;
+            }
+        }
+    }
+    __syncthreads();
+    u32* dst = out + (size_t)n0 * L;
+    for (int idx = (int)threadIdx.x; idx < cnt * L; idx += (int)blockDim.x) {
+        const int c = idx / L, l = idx - c * L;
+        dst[idx] = Z[(size_t)c * ZL + l];
+    }
+}
+
+// ---------------------------------------------------------------------------
+// template dispatch
+// ---------------------------------------------------------------------------
+template <bool INV>
+hipError_t col_dispatch(int nb, dim3 grid, dim3 block, size_t lds, hipStream_t s, const DevPlan& D, u32* R0, u32* R1,
+                        int tc) {
+    switch (nb) {
+#define PM_COL(n) \
+    case n: hipLaunchKernelGGL((k_col<n, INV>), grid, block, lds, s, D, R0, R1, tc); break;
+        PM_COL(1) PM_COL(2) PM_COL(3) PM_COL(4) PM_COL(5) PM_COL(6) PM_COL(7) PM_COL(8)
+#undef PM_COL
+        default: return hipErrorInvalidValue;
+    }
+    return hipGetLastError();
+}
+
+template <int MODE>
+hipError_t row_dispatch(int nb, dim3 grid, dim3 block, size_t lds, hipStream_t s, const DevPlan& D, u32* RB,
+                        const u32* RA, int rpb) {
+    switch (nb) {
+#define PM_ROW(n) \
+    case n: hipLaunchKernelGGL((k_row<n, MODE>), grid, block, lds, s, D, RB, RA, rpb); break;
+        PM_ROW(0) PM_ROW(1) PM_ROW(2) PM_ROW(3) PM_ROW(4) PM_ROW(5) PM_ROW(6) PM_ROW(7) PM_ROW(8)
+#undef PM_ROW
+        default: return hipErrorInvalidValue;
+    }
+    return hipGetLastError();
+}
+
+int line_threads(int nb) { return nb >= 4 ? 1 << (nb - 4) : 1; }
+
+}  // namespace
+
+LaunchCfg choose_launch(const DevPlan& D, size_t max_lds) {
+    LaunchCfg C;
+    const u32 nrows = 1u << D.w1, ncols = 1u << D.w2;
+    C.rns_stride = 32 * D.rns_ks + 16;
+    C.rns_tn = 128;
+    while (C.rns_tn > 32 && (size_t)C.rns_tn * C.rns_stride > max_lds) C.rns_tn /= 2;
+    C.rns_lds = (size_t)C.rns_tn * C.rns_stride;
+    C.crt_ys = 32 * D.crt_ks + 16;
+    C.crt_tn = 128;
+    auto crt_need = [&](int tn) { return (size_t)tn * C.crt_ys + (size_t)tn * (D.L + 3) * 8; };
+    while (C.crt_tn > 32 && crt_need(C.crt_tn) > max_lds) C.crt_tn /= 2;
+    C.crt_lds = crt_need(C.crt_tn);
+    C.col_tc = (int)std::min<u32>(ncols, 32);
+    C.row_rpb = (int)std::min<u32>(nrows, (u32)(256 / line_threads(D.w2)));
+    return C;
+}
+
+hipError_t launch_rns(const DevPlan& D, const LaunchCfg& C, const u32* a0, const u32* a1, u32* R0, u32* R1, int nops,
+                      hipStream_t s) {
+    const dim3 grid((D.N + C.rns_tn - 1) / C.rns_tn, nops), block(256);
+    switch (C.rns_tn) {
+        case 128: hipLaunchKernelGGL((k_rns<4>), grid, block, C.rns_lds, s, D, a0, a1, R0, R1, C.rns_stride); break;
+        case 64: hipLaunchKernelGGL((k_rns<2>), grid, block, C.rns_lds, s, D, a0, a1, R0, R1, C.rns_stride); break;
+        case 32: hipLaunchKernelGGL((k_rns<1>), grid, block, C.rns_lds, s, D, a0, a1, R0, R1, C.rns_stride); break;
+        default: return hipErrorInvalidValue;
+    }
+    return hipGetLastError();
+}
+
+hipError_t launch_col(const DevPlan& D, const LaunchCfg& C, u32* R0, u32* R1, int nops, bool inverse, hipStream_t s) {
+    if (D.w1 == 0) return hipSuccess;
+    const u32 ncols = 1u << D.w2;
+    const int T = line_threads(D.w1);
+    const dim3 grid(ncols / C.col_tc, D.K, nops), block(C.col_tc * T);
+    const size_t lds = D.w1 > 4 ? ((size_t)1 << D.w1) * C.col_tc * 4 : 0;
+    return inverse ? col_dispatch<true>(D.w1, grid, block, lds, s, D, R0, R1, C.col_tc)
+                   : col_dispatch<false>(D.w1, grid, block, lds, s, D, R0, R1, C.col_tc);
+}
+
+hipError_t launch_row(const DevPlan& D, const LaunchCfg& C, u32* RB, const u32* RA, RowMode mode, hipStream_t s) {
+    const u32 nrows = 1u << D.w1;
+    const int T = line_threads(D.w2);
+    const dim3 grid(nrows / C.row_rpb, D.K), block(C.row_rpb * T);
+    const size_t rs = ((size_t)1 << D.w2) + (D.w2 > 4 ? 16 : 0);
+    const size_t lds = D.w2 > 4 ? 2 * (size_t)C.row_rpb * rs * 4 : 0;
+    switch (mode) {
+        case ROW_FUSED: return row_dispatch<ROW_FUSED>(D.w2, grid, block, lds, s, D, RB, RA, C.row_rpb);
+        case ROW_FWD: return row_dispatch<ROW_FWD>(D.w2, grid, block, lds, s, D, RB, RA, C.row_rpb);
+        case ROW_INV: return row_dispatch<ROW_INV>(D.w2, grid, block, lds, s, D, RB, RA, C.row_rpb);
+    }
+    return hipErrorInvalidValue;
+}
+
+hipError_t launch_crt(const DevPlan& D, const LaunchCfg& C, const u32* R, u32* out, const uint2* G, hipStream_t s) {
+    const dim3 grid((D.N + C.crt_tn - 1) / C.crt_tn), block(256);
+    switch (C.crt_tn) {
+        case 128: hipLaunchKernelGGL((k_crt<4>), grid, block, C.crt_lds, s, D, R, out, G, C.crt_ys); break;
+        case 64: hipLaunchKernelGGL((k_crt<2>), grid, block, C.crt_lds, s, D, R, out, G, C.crt_ys); break;
+        case 32: hipLaunchKernelGGL((k_crt<1>), grid, block, C.crt_lds, s, D, R, out, G, C.crt_ys); break;
+        default: return hipErrorInvalidValue;
+    }
+    return hipGetLastError();
+}
+
+}  // namespace pm

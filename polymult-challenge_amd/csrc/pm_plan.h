@@ -1,0 +1,55 @@
+// Host-side plan for one point (N, W, L, q): validation, RNS primes and every
+// constant table the device kernels read. Pure C++, no HIP: the tables are
+// uploaded by pm_api.hip.
+#pragma once
+
+#include <cstddef>
+#include <cstdint>
+#include <vector>
+
+namespace pm {
+
+// Limits of this build. N above 2^16 would need a third NTT pass.
+constexpr int kMaxLogN = 16;
+constexpr int kMaxK = 1023;
+
+struct HostPlan {
+    uint32_t N = 0, W = 0, L = 0;
+    int logN = 0, w1 = 0, w2 = 0;  // 4-step split: nrows = 2^w1, ncols = 2^w2
+    int K = 0;                      // RNS primes
+    uint64_t delta = 0;             // 2^W - q
+    std::vector<uint32_t> q;        // [L]
+    std::vector<uint32_t> p, pinv;  // [K]; pinv = -p^-1 mod 2^32
+    std::vector<uint32_t> psi;      // [K] primitive 2N-th roots
+
+    // NTT: tw[j][k] = (w, w') with w = psi_j^bitrev_logN(k) mod p_j and the
+    // Shoup companion w' = floor(w 2^32 / p_j). [K][N][2]
+    std::vector<uint32_t> tw;
+
+    // RNS GEMM (int8 MFMA, 32x32x32): A fragments in lane order,
+    // [rns_rt][rns_ks][64 lanes][16 bytes]. Row m = 4 j + t holds the
+    // balanced base-256 digit t of c_jb = 2^(8b+32) mod p_j (symmetric
+    // representative), column k = byte b of the coefficient.
+    std::vector<int8_t> rns_frag;
+    std::vector<uint64_t> rns_bias;  // [K] 128 sum_b c_jb mod p_j + p_j 2^20
+    int rns_rt = 0, rns_ks = 0;
+
+    // CRT GEMM: rows are digit positions of z (4L + 4 of them) and of the
+    // v estimate F (6, padded to 8); columns k = 4 j + s index byte s of y_j.
+    std::vector<int8_t> crt_frag;    // [crt_rt][crt_ks][64][16]
+    int crt_rt = 0, crt_ks = 0;
+    std::vector<uint32_t> G;         // [K][2] Shoup pair of N^-1 2^32 (M/p_j)^-1
+    std::vector<uint32_t> Gplain;    // [K][2] Shoup pair of (M/p_j)^-1 (stage tests)
+    std::vector<uint32_t> C0;        // [L + 2] 128 * 0x01010101 * sum_j Q_j
+    int64_t F0 = 0;                  // 128 * 0x01010101 * sum_j floor(2^48 / p_j)
+    std::vector<uint32_t> T;         // [K + 1][L] (-v M) mod q
+};
+
+// Validates the point and chooses the primes. Throws std::runtime_error with
+// the reference's messages (solve.cu:873-913).
+void plan_point(HostPlan& P, uint32_t N, uint32_t W, uint32_t L, const uint32_t* q, size_t q_limbs);
+
+// Builds the NTT, RNS and CRT tables of a planned point.
+void build_tables(HostPlan& P);
+
+}  // namespace pm

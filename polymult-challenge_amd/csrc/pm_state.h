@@ -1,0 +1,44 @@
+// Per-point state behind the C ABI.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstddef>
+#include <cstdint>
+
+#include "pm_kernels.h"
+#include "pm_plan.h"
+
+namespace pm {
+
+struct Pipeline;  // pm_pipeline.cpp: pinned staging, streams, host workers
+
+struct State {
+    HostPlan host;
+    DevPlan dev{};
+    LaunchCfg cfg;
+    int device = 0;
+    hipStream_t stream = nullptr;  // compute stream
+    void* d_tables = nullptr;
+    size_t table_bytes = 0;
+    uint32_t* d_R[2] = {nullptr, nullptr};   // residues [K][N] of a and b
+    uint32_t* d_in[2] = {nullptr, nullptr};  // operands [N][L] for host calls
+    uint32_t* d_out = nullptr;               // result [N][L] for host calls
+    size_t coeff_words = 0;                  // N * L
+    size_t device_bytes = 0;
+    Pipeline* pipe = nullptr;
+
+    void init(uint32_t N, uint32_t W, uint32_t L, const uint32_t* q, size_t q_limbs);
+    void upload_plan();
+    // The whole multiply on device buffers, enqueued on st.
+    hipError_t enqueue_multiply(const uint32_t* a, const uint32_t* b, uint32_t* c, hipStream_t st);
+
+    void pipeline_init();
+    void pipeline_free();
+    // c = a b from host buffers through the PCIe pipeline; blocking.
+    void run_host(const uint32_t* a, const uint32_t* b, uint32_t* c);
+
+    ~State();
+};
+
+}  // namespace pm

@@ -1,0 +1,220 @@
+"""ctypes binding of libpolymult.so (include/polymult.h).
+
+The Python face of the drop-in: the same three calls as the reference's
+fherma_init / fherma_run / fherma_free (fherma.h:46-48), here
+Multiplier(N, W, q) / Multiplier.run(a, b) / Multiplier.close(), with the
+same argument meaning (N x L little-endian u32 limbs) and the same error
+behaviour (a bad point or operand raises with the reference's message).
+
+There is no CPU fallback: without the built library every call raises.
+"""
+from __future__ import annotations
+
+import ctypes
+from pathlib import Path
+
+import numpy as np
+
+HERE = Path(__file__).resolve().parent
+LIB_PATH = HERE / "libpolymult.so"
+
+# Every C symbol include/polymult.h declares.
+C_SYMBOLS = (
+    "pm_init", "pm_run", "pm_run_device", "pm_free", "pm_last_error", "pm_query", "pm_point_info",
+    "pm_dev_alloc", "pm_dev_free", "pm_copy_to_device", "pm_copy_to_host", "pm_sync", "pm_time_device",
+    "pm_stage_rns", "pm_stage_ntt_fwd", "pm_stage_ntt_inv", "pm_stage_crt",
+)
+# The C++ entry points of include/fherma.h, mangled as the reference's harness binds them.
+CXX_SYMBOLS = (
+    "_Z11fherma_initRKN6fherma5PointE",
+    "_Z10fherma_runPvRKN6fherma6InputsE",
+    "_Z11fherma_freePv",
+)
+
+
+class PolymultError(RuntimeError):
+    pass
+
+
+class PmInfo(ctypes.Structure):
+    _fields_ = [("N", ctypes.c_uint32), ("W", ctypes.c_uint32), ("L", ctypes.c_uint32), ("K", ctypes.c_uint32),
+                ("delta", ctypes.c_uint64), ("w1", ctypes.c_uint32), ("w2", ctypes.c_uint32),
+                ("device_bytes", ctypes.c_uint64)]
+
+
+_LIB = None
+_u32p = ctypes.POINTER(ctypes.c_uint32)
+_vp = ctypes.c_void_p
+
+
+def load() -> ctypes.CDLL:
+    global _LIB
+    if _LIB is not None:
+        return _LIB
+    if not LIB_PATH.exists():
+        raise PolymultError("libpolymult.so is not built (run __graft_entry__.build())")
+    lib = ctypes.CDLL(str(LIB_PATH))
+    sig = {
+        "pm_init": ([ctypes.c_uint32, ctypes.c_uint32, ctypes.c_uint32, _u32p, ctypes.c_size_t,
+                     ctypes.POINTER(_vp)], ctypes.c_int),
+        "pm_run": ([_vp, _u32p, _u32p, _u32p], ctypes.c_int),
+        "pm_run_device": ([_vp, _vp, _vp, _vp, _vp], ctypes.c_int),
+        "pm_free": ([_vp], None),
+        "pm_last_error": ([], ctypes.c_char_p),
+        "pm_query": ([_vp, ctypes.POINTER(PmInfo)], ctypes.c_int),
+        "pm_point_info": ([ctypes.c_uint32, ctypes.c_uint32, ctypes.c_uint32, _u32p, ctypes.c_size_t, _u32p,
+                           ctypes.POINTER(ctypes.c_uint64), _u32p, ctypes.c_size_t], ctypes.c_int),
+        "pm_dev_alloc": ([ctypes.c_size_t, ctypes.POINTER(_vp)], ctypes.c_int),
+        "pm_dev_free": ([_vp], ctypes.c_int),
+        "pm_copy_to_device": ([_vp, _vp, ctypes.c_size_t], ctypes.c_int),
+        "pm_copy_to_host": ([_vp, _vp, ctypes.c_size_t], ctypes.c_int),
+        "pm_sync": ([_vp], ctypes.c_int),
+        "pm_time_device": ([_vp, _vp, _vp, _vp, ctypes.c_int, ctypes.POINTER(ctypes.c_float)], ctypes.c_int),
+        "pm_stage_rns": ([_vp, _vp, _vp], ctypes.c_int),
+        "pm_stage_ntt_fwd": ([_vp, _vp], ctypes.c_int),
+        "pm_stage_ntt_inv": ([_vp, _vp], ctypes.c_int),
+        "pm_stage_crt": ([_vp, _vp, _vp], ctypes.c_int),
+    }
+    for name, (args, res) in sig.items():
+        f = getattr(lib, name)
+        f.argtypes = args
+        f.restype = res
+    _LIB = lib
+    return lib
+
+
+def _check(rc: int):
+    if rc != 0:
+        raise PolymultError(load().pm_last_error().decode())
+
+
+def _u32(a: np.ndarray) -> np.ndarray:
+    a = np.ascontiguousarray(a, dtype=np.uint32)
+    return a
+
+
+def _ptr(a: np.ndarray):
+    return a.ctypes.data_as(_u32p)
+
+
+def point_info(N: int, W: int, L: int, q: np.ndarray, cap: int = 1024):
+    """Host-only validation: (K, delta, primes). Raises like pm_init."""
+    lib = load()
+    q = _u32(q)
+    K = ctypes.c_uint32()
+    delta = ctypes.c_uint64()
+    primes = np.zeros(cap, dtype=np.uint32)
+    _check(lib.pm_point_info(N, W, L, _ptr(q), q.size, ctypes.byref(K), ctypes.byref(delta), _ptr(primes), cap))
+    return K.value, delta.value, primes[:K.value].copy()
+
+
+class DeviceBuffer:
+    """Device memory owned through libpolymult's own HIP runtime."""
+
+    def __init__(self, nbytes: int):
+        self.nbytes = nbytes
+        p = _vp()
+        _check(load().pm_dev_alloc(nbytes, ctypes.byref(p)))
+        self.ptr = p
+
+    @classmethod
+    def from_array(cls, a: np.ndarray) -> "DeviceBuffer":
+        a = np.ascontiguousarray(a)
+        buf = cls(a.nbytes)
+        buf.upload(a)
+        return buf
+
+    def upload(self, a: np.ndarray):
+        a = np.ascontiguousarray(a)
+        assert a.nbytes <= self.nbytes
+        _check(load().pm_copy_to_device(self.ptr, a.ctypes.data_as(_vp), a.nbytes))
+
+    def download(self, shape, dtype=np.uint32) -> np.ndarray:
+        out = np.empty(shape, dtype=dtype)
+        _check(load().pm_copy_to_host(out.ctypes.data_as(_vp), self.ptr, out.nbytes))
+        return out
+
+    def free(self):
+        if self.ptr:
+            _check(load().pm_dev_free(self.ptr))
+            self.ptr = None
+
+    def __del__(self):
+        try:
+            self.free()
+        except Exception:
+            pass
+
+
+class Multiplier:
+    """One point (N, W, q): the fherma_init / fherma_run / fherma_free triple."""
+
+    def __init__(self, N: int, W: int, q: np.ndarray, L: int | None = None):
+        lib = load()
+        L = (W + 31) // 32 if L is None else L
+        q = _u32(q).reshape(-1)
+        h = _vp()
+        _check(lib.pm_init(N, W, L, _ptr(q), q.size, ctypes.byref(h)))
+        self._h = h
+        info = PmInfo()
+        _check(lib.pm_query(h, ctypes.byref(info)))
+        self.info = info
+        self.N, self.W, self.L, self.K = info.N, info.W, info.L, info.K
+
+    def close(self):
+        if getattr(self, "_h", None):
+            load().pm_free(self._h)
+            self._h = None
+
+    def __del__(self):
+        self.close()
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    # -- the multiply ---------------------------------------------------------
+    def run(self, a: np.ndarray, b: np.ndarray, out: np.ndarray | None = None) -> np.ndarray:
+        a, b = _u32(a), _u32(b)
+        words = self.N * self.L
+        if a.size != words or b.size != words:
+            raise PolymultError("operand size does not match N x L")
+        c = np.empty((self.N, self.L), dtype=np.uint32) if out is None else out
+        _check(load().pm_run(self._h, _ptr(a), _ptr(b), _ptr(c)))
+        return c
+
+    def run_device(self, da: DeviceBuffer, db: DeviceBuffer, dc: DeviceBuffer, stream=None):
+        _check(load().pm_run_device(self._h, da.ptr, db.ptr, dc.ptr, stream))
+
+    def sync(self):
+        _check(load().pm_sync(self._h))
+
+    def time_device(self, da: DeviceBuffer, db: DeviceBuffer, dc: DeviceBuffer, iters: int) -> list[float]:
+        ms = (ctypes.c_float * 6)()
+        _check(load().pm_time_device(self._h, da.ptr, db.ptr, dc.ptr, iters, ms))
+        return list(ms)
+
+    # -- stage entry points ---------------------------------------------------
+    def stage_rns(self, a: np.ndarray) -> np.ndarray:
+        da = DeviceBuffer.from_array(_u32(a))
+        dR = DeviceBuffer(self.K * self.N * 4)
+        _check(load().pm_stage_rns(self._h, da.ptr, dR.ptr))
+        return dR.download((self.K, self.N))
+
+    def stage_ntt_fwd(self, R: np.ndarray) -> np.ndarray:
+        dR = DeviceBuffer.from_array(_u32(R))
+        _check(load().pm_stage_ntt_fwd(self._h, dR.ptr))
+        return dR.download((self.K, self.N))
+
+    def stage_ntt_inv(self, R: np.ndarray) -> np.ndarray:
+        dR = DeviceBuffer.from_array(_u32(R))
+        _check(load().pm_stage_ntt_inv(self._h, dR.ptr))
+        return dR.download((self.K, self.N))
+
+    def stage_crt(self, X: np.ndarray) -> np.ndarray:
+        dX = DeviceBuffer.from_array(_u32(X))
+        dc = DeviceBuffer(self.N * self.L * 4)
+        _check(load().pm_stage_crt(self._h, dX.ptr, dc.ptr))
+        return dc.download((self.N, self.L))

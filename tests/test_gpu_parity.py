@@ -1,0 +1,160 @@
+"""Parity of the HIP path with the oracle, stage by stage and end to end.
+
+Every comparison is bit-exact (integer work). Sizes where the C oracle takes
+seconds are compared directly; the full configurations are compared directly
+too (the C oracle takes ~2 s at N=65536, W=1747) and through the closed form
+for all-(q-1) operands and sparse brute force.
+"""
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+# The reference's emulator points (tests/run_emu.sh:16-29) plus small edge cases.
+EMU_POINTS = [(64, 32), (16, 33), (2, 40), (1024, 512), (256, 1747), (4096, 100), (8192, 256), (2048, 868),
+              (512, 4000)]
+EDGE_POINTS = [(1, 8), (2, 16), (4, 20), (8, 33), (16, 64), (32, 100), (128, 256), (2, 3), (32, 31)]
+KINDS = ["uniform", "max", "sparse", "small16", "mixed", "zero"]
+
+
+def _setup(O, pm, N, W):
+    q = O.spec_q(N, W)
+    L = O.limbs_of(W)
+    return q, L, pm.Multiplier(N, W, O.q_limbs(q, L)), O.OraclePlan(N, W, q)
+
+
+def _diff(c, want):
+    return int(np.count_nonzero((np.asarray(c) != np.asarray(want)).reshape(len(want), -1).any(axis=1)))
+
+
+@pytest.mark.parametrize("N,W", [(64, 32), (1024, 512), (4096, 256), (2048, 868), (512, 4000), (65536, 1747)])
+def test_stage_rns(O, pm, N, W):
+    q, L, m, P = _setup(O, pm, N, W)
+    print("stage rns N=%d W=%d K=%d" % (N, W, m.K), flush=True)
+    assert m.K == P.K
+    for kind in ("uniform", "max", "mixed"):
+        a, _ = O.operands(kind, N, W, q, 11)
+        got = m.stage_rns(a)
+        want = P.rns(a)
+        assert np.array_equal(got, want), "%s: %d residues differ" % (kind, int(np.count_nonzero(got != want)))
+
+
+@pytest.mark.parametrize("N,W", [(1, 8), (2, 16), (4, 20), (16, 33), (64, 32), (256, 1747), (1024, 512),
+                                 (4096, 256), (8192, 256), (65536, 1747)])
+def test_stage_ntt(O, pm, N, W):
+    q, L, m, P = _setup(O, pm, N, W)
+    print("stage ntt N=%d W=%d K=%d" % (N, W, m.K), flush=True)
+    rng = np.random.default_rng(N + W)
+    p = P.primes.astype(np.uint64)[:, None]
+    R = (rng.integers(0, 1 << 62, size=(m.K, N), dtype=np.uint64) % p).astype(np.uint32)
+    fwd = m.stage_ntt_fwd(R)
+    assert np.array_equal(fwd, P.ntt_fwd(R))
+    inv = m.stage_ntt_inv(R)
+    want = (P.ntt_inv(R).astype(np.uint64) * np.uint64(N) % p).astype(np.uint32)  # unscaled inverse
+    assert np.array_equal(inv, want)
+
+
+@pytest.mark.parametrize("N,W", [(1, 8), (64, 32), (1024, 512), (2048, 868), (512, 4000), (65536, 1747)])
+def test_stage_crt(O, pm, N, W):
+    q, L, m, P = _setup(O, pm, N, W)
+    print("stage crt N=%d W=%d" % (N, W), flush=True)
+    for kind in ("uniform", "max", "small16"):
+        a, b = O.operands(kind, N, W, q, 5)
+        X = P.ntt_inv(P.pointwise(P.ntt_fwd(P.rns(a)), P.ntt_fwd(P.rns(b))))
+        got = m.stage_crt(X)
+        want = P.crt(X)
+        assert np.array_equal(got, want), "%s: %d coefficients differ" % (kind, _diff(got, want))
+
+
+@pytest.mark.parametrize("N,W", EMU_POINTS + EDGE_POINTS)
+def test_multiply_points(O, pm, N, W):
+    q, L, m, P = _setup(O, pm, N, W)
+    print("multiply N=%d W=%d K=%d" % (N, W, m.K), flush=True)
+    for s, kind in enumerate(KINDS):
+        a, b = O.operands(kind, N, W, q, 1000 + 7919 * s + N + W)
+        got = m.run(a, b)
+        want = P.multiply(a, b)
+        assert np.array_equal(got, want), "%s: %d of %d coefficients differ" % (kind, _diff(got, want), N)
+
+
+def test_multiply_schoolbook_small(O, pm):
+    """Independent of the oracle's NTT: schoolbook negacyclic products."""
+    for N, W in [(1, 8), (8, 33), (64, 32), (128, 256)]:
+        q, L, m, _ = _setup(O, pm, N, W)
+        for seed in range(3):
+            a, b = O.operands("uniform", N, W, q, seed)
+            assert O.unpack(m.run(a, b)) == O.schoolbook(O.unpack(a), O.unpack(b), q)
+
+
+def test_multiply_kronecker(O, pm):
+    """Independent of the oracle's NTT: Kronecker-substitution bigint product."""
+    for N, W in [(4096, 256), (1024, 868)]:
+        q, L, m, _ = _setup(O, pm, N, W)
+        a, b = O.operands("mixed", N, W, q, 77)
+        assert O.unpack(m.run(a, b)) == O.kronecker(O.unpack(a), O.unpack(b), q)
+
+
+# ---- the measured configurations ----
+
+def test_config_4096_256(O, pm):
+    q, L, m, P = _setup(O, pm, 4096, 256)
+    assert (m.K, (1 << 256) - q) == (18, 466943)
+    for kind in ("uniform", "max", "sparse", "small16", "mixed"):
+        a, b = O.operands(kind, 4096, 256, q, 4096)
+        assert np.array_equal(m.run(a, b), P.multiply(a, b)), kind
+
+
+def test_config_32768_868(O, pm):
+    N, W = 32768, 868
+    q, L, m, P = _setup(O, pm, N, W)
+    assert (m.K, (1 << W) - q) == (57, 11993087)
+    for kind in ("uniform", "max", "sparse", "small16"):
+        a, b = O.operands(kind, N, W, q, 32768)
+        got = m.run(a, b)
+        assert np.array_equal(got, P.multiply(a, b)), kind
+        if kind == "max":
+            assert O.unpack(got) == O.all_qm1(N, q)
+
+
+def test_config_65536_1747(O, pm):
+    N, W = 65536, 1747
+    q, L, m, P = _setup(O, pm, N, W)
+    assert (m.K, (1 << W) - q) == (114, 9699327)
+    a, b = O.operands("mixed", N, W, q, 65536)
+    got = m.run(a, b)
+    assert np.array_equal(got, P.multiply(a, b))
+    # closed form, independent of every NTT detail
+    am = np.tile(O.q_limbs(q - 1, L), (N, 1))
+    assert O.unpack(m.run(am, am)) == O.all_qm1(N, q)
+    # commutativity and device-resident path
+    assert np.array_equal(m.run(b, a), got)
+    da, db = pm.DeviceBuffer.from_array(a), pm.DeviceBuffer.from_array(b)
+    dc = pm.DeviceBuffer(N * L * 4)
+    m.run_device(da, db, dc)
+    m.sync()
+    assert np.array_equal(dc.download((N, L)), got)
+
+
+def test_sparse_bruteforce_65536(O, pm):
+    """Sparse operands against O(nnz^2) monomial brute force at full size."""
+    N, W = 65536, 1747
+    q, L, m, _ = _setup(O, pm, N, W)
+    rng = np.random.default_rng(9)
+    a = np.zeros((N, L), dtype=np.uint32)
+    b = np.zeros((N, L), dtype=np.uint32)
+    ia = rng.choice(N, 24, replace=False)
+    ib = rng.choice(N, 24, replace=False)
+    a[ia] = O.uniform_below_q(rng, 24, W, q)
+    b[ib] = O.uniform_below_q(rng, 24, W, q)
+    da = {int(i): v for i, v in zip(ia, O.unpack(a[ia]))}
+    db = {int(i): v for i, v in zip(ib, O.unpack(b[ib]))}
+    assert O.unpack(m.run(a, b)) == O.sparse_product(da, db, N, q)
+
+
+def test_errors_like_reference(O, pm):
+    q = O.spec_q(64, 32)
+    with pytest.raises(pm.PolymultError, match="power of two"):
+        pm.Multiplier(48, 32, O.q_limbs(q, 1))
+    m = pm.Multiplier(64, 32, O.q_limbs(q, 1))
+    with pytest.raises(pm.PolymultError, match="operand size"):
+        m.run(np.zeros(10, np.uint32), np.zeros(10, np.uint32))
