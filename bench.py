@@ -1,0 +1,177 @@
+"""Benchmark of the negacyclic multiply c = a b mod (X^N + 1, q) on one MI355X.
+
+Workload: the plain single product at N=65536, W=1747 (L=55 limbs, K=114 RNS
+primes, q = 2^1747 - 9699327), operands mixed per coefficient (1/2 uniform in
+[0, q), 1/4 q-1, 1/4 below 2^32), seeded. One step = one whole multiply with
+a and b already resident in HBM (RNS, three NTT passes, CRT), launched
+through the C ABI (pm_run_device). `value` is multiplies per second over the
+timed steps. The PCIe-inclusive rate of the host-buffer call (pm_run, the
+fherma_run replacement) is reported beside it, never as `value`.
+
+    python bench.py [--gpus 1] [--steps 200] [--warmup 20] [--no-cpu]
+
+Prints one JSON line.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import subprocess
+import sys
+import tempfile
+import time
+from pathlib import Path
+
+import numpy as np
+
+ROOT = Path(__file__).resolve().parent
+sys.path.insert(0, str(ROOT / "polymult-challenge_amd"))
+sys.path.insert(0, str(ROOT))
+
+N, W = 65536, 1747
+SEED = 20260
+HBM_PEAK_GBS = 8000.0
+PCIE_PEAK_GBS = 64.0  # PCIe Gen5 x16, per direction (vendor figure)
+METRIC = "mults/s at N=65536/W=1747 (plain vs batched), %HBM/PCIe peak, serve p99 ms"
+KERNELS = ["rns", "col_fwd", "row_fused", "col_inv", "crt"]
+
+
+def kernel_bytes(K: int, L: int) -> dict:
+    """Algorithmic HBM bytes of each kernel per multiply: the operands,
+    residues and result it must read and write once (no twiddles, no
+    re-reads)."""
+    r = K * N * 4  # one residue array
+    c = N * L * 4  # one coefficient array
+    return {"rns": 2 * c + 2 * r, "col_fwd": 4 * r, "row_fused": 3 * r, "col_inv": 2 * r, "crt": r + c}
+
+
+def modmuls(K: int) -> dict:
+    """Modular multiplications per kernel per multiply (butterflies count one)."""
+    logn = N.bit_length() - 1
+    half = N // 2
+    w1 = logn // 2
+    w2 = logn - w1
+    return {"rns": 0, "col_fwd": 2 * K * half * w1, "row_fused": K * (3 * half * w2 + N),
+            "col_inv": K * half * w1, "crt": K * N}
+
+
+def cpu_baseline(q: int, sample_cases: int) -> dict | None:
+    """The reference's own solve.cu on its CPU emulator (oracle/_ref), pinned
+    to one core, on `sample_cases` products of this workload."""
+    from oracle import oracle as O
+
+    exe = O.ref_binary()
+    if exe is None:
+        return None
+    with tempfile.TemporaryDirectory() as d:
+        cases = [O.operands("mixed", N, W, q, SEED + 1 + i) for i in range(sample_cases)]
+        O.write_point(d, N, W, q, cases)
+        core = sorted(os.sched_getaffinity(0))[0]
+        t0 = time.perf_counter()
+        subprocess.run([str(exe), d], check=True, capture_output=True,
+                       preexec_fn=lambda: os.sched_setaffinity(0, {core}))
+        wall = time.perf_counter() - t0
+        res = O.read_results(d)
+        secs = [c["seconds"] for c in res["cases"] if c["status"] == "ok"]
+        # spot-check one output against the oracle
+        L = O.limbs_of(W)
+        ok = bool(np.array_equal(O.read_output(d, 0, N, L), O.OraclePlan(N, W, q).multiply(*cases[0])))
+    total = sum(secs)
+    return {"value": len(secs) / total if total else None, "unit": "mults/s", "cores": 1, "kind": "reference",
+            "sample": "%d products at N=65536/W=1747 (mixed operands), fherma_run time from the reference's "
+                      "results.json; solve.cu on pm_emu.h built by oracle/Makefile, pinned to one core "
+                      "(wall %.1f s incl. init)" % (len(secs), wall),
+            "per_call_s": secs, "output_matches_oracle": ok}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=200)
+    ap.add_argument("--warmup", type=int, default=20)
+    ap.add_argument("--e2e-calls", type=int, default=20)
+    ap.add_argument("--cpu-cases", type=int, default=3)
+    ap.add_argument("--no-cpu", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print("bench.py measures one GPU: this build runs on a single MI355X (the path would shard by RNS "
+              "prime across GPUs, see the design notes); --gpus %d is not supported" % args.gpus, file=sys.stderr)
+        sys.exit(2)
+
+    import pmlib
+    from oracle import oracle as O
+
+    q = O.spec_q(N, W)
+    L = O.limbs_of(W)
+    a, b = O.operands("mixed", N, W, q, SEED)
+    m = pmlib.Multiplier(N, W, O.q_limbs(q, L))
+    K = m.K
+    da, db = pmlib.DeviceBuffer.from_array(a), pmlib.DeviceBuffer.from_array(b)
+    dc = pmlib.DeviceBuffer(N * L * 4)
+
+    # correctness of the measured configuration, once, before timing
+    m.run_device(da, db, dc)
+    m.sync()
+    c_dev = dc.download((N, L))
+    correct = bool(np.array_equal(c_dev, O.OraclePlan(N, W, q).multiply(a, b)))
+
+    for _ in range(args.warmup):
+        m.run_device(da, db, dc)
+    m.sync()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        m.run_device(da, db, dc)
+    m.sync()
+    dt = time.perf_counter() - t0
+    ms_step = 1e3 * dt / args.steps
+    value = args.steps / dt
+
+    # per-kernel times from HIP events on the stream the kernels run on
+    ms = m.time_device(da, db, dc, max(10, min(args.steps, 50)))
+    kms = dict(zip(KERNELS, ms[1:]))
+    kb = kernel_bytes(K, L)
+    dom = max(KERNELS, key=lambda k: kms[k])
+    ach = kb[dom] / (kms[dom] * 1e-3) / 1e9
+    traffic = None
+    tf = ROOT / "profiles" / "traffic.json"
+    if tf.exists():
+        traffic = json.loads(tf.read_text()).get(dom)
+    roofline = {"bound": "hbm", "kernel": dom, "achieved": round(ach, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                "frac": round(ach / HBM_PEAK_GBS, 4), "traffic": traffic,
+                "algorithmic_bytes": kb[dom], "kernel_ms": round(kms[dom], 4)}
+    mm = modmuls(K)
+    per_kernel = {k: {"ms": round(kms[k], 4), "GBps": round(kb[k] / (kms[k] * 1e-3) / 1e9, 1),
+                      "Gmodmul_per_s": round(mm[k] / (kms[k] * 1e-3) / 1e9, 1)} for k in KERNELS}
+
+    # end to end from pageable host buffers (the fherma_run replacement)
+    c_host = np.empty((N, L), np.uint32)
+    m.run(a, b, c_host)
+    e2e = []
+    for _ in range(args.e2e_calls):
+        t = time.perf_counter()
+        m.run(a, b, c_host)
+        e2e.append(time.perf_counter() - t)
+    e2e_med = float(np.median(e2e))
+    pcie_bytes = 12 * N * L
+    end_to_end = {"mults_per_s": round(1.0 / e2e_med, 2), "median_ms": round(1e3 * e2e_med, 3),
+                  "pct_pcie_peak": round(100 * pcie_bytes / e2e_med / (PCIE_PEAK_GBS * 1e9), 1),
+                  "bit_exact": bool(np.array_equal(c_host, c_dev))}
+
+    cpu = None if args.no_cpu else cpu_baseline(q, args.cpu_cases)
+    out = {
+        "metric": METRIC, "value": round(value, 2), "unit": "mults/s", "n_gpus": 1, "steps": args.steps,
+        "warmup": args.warmup, "ms_per_step": round(ms_step, 4), "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": "u32", "data": "synthetic",
+        "config": {"workload": "plain single product N=65536 W=1747 L=55 K=%d, mixed operands, "
+                               "device-resident (pm_run_device)" % K, "N": N, "W": W, "L": L, "K": K,
+                   "operands": "mixed per coefficient: 1/2 uniform [0,q), 1/4 q-1, 1/4 < 2^32", "seed": SEED},
+        "bit_exact": correct,
+        "pct_hbm_peak": round(100 * pcie_bytes / (ms_step * 1e-3) / (HBM_PEAK_GBS * 1e9), 3),
+        "roofline": roofline, "kernels": per_kernel, "end_to_end": end_to_end, "cpu_baseline": cpu,
+    }
+    print(json.dumps(out))
+
+
+if __name__ == "__main__":
+    main()
