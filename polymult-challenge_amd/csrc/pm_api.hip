@@ -85,6 +85,7 @@ void State::upload_plan() {
     D.rns_bias = reinterpret_cast<const uint64_t*>(at(o_rb));
     D.rns_rt = H.rns_rt;
     D.rns_ks = H.rns_ks;
+    D.magic_L = H.L > 1 ? (uint32_t)((0x100000000ull + H.L - 1) / H.L) : 0u;
     D.crt_frag = reinterpret_cast<const v4i*>(at(o_cf));
     D.crt_rt = H.crt_rt;
     D.crt_ks = H.crt_ks;

@@ -20,6 +20,7 @@ struct DevPlan {
     const v4i* rns_frag;   // [rns_rt][rns_ks][64]
     const uint64_t* rns_bias;
     int rns_rt, rns_ks;
+    uint32_t magic_L;      // ceil(2^32 / L) for L >= 2: idx / L = umulhi(idx, magic_L) for idx < 2^20
     const v4i* crt_frag;   // [crt_rt][crt_ks][64]
     int crt_rt, crt_ks;
     const uint2* G;        // [K]

@@ -284,20 +284,50 @@ __global__ void __launch_bounds__(256) k_row(DevPlan D, u32* __restrict__ RB, co
 }
 
 // ---------------------------------------------------------------------------
-// RNS decomposition. Block = 4 waves and 32 NCT coefficients, staged in LDS
-// as bytes (row stride `stride`). The GEMM is C[(j,t)][n] = sum_b D[(j,t)][b]
-// (byte_b(a_n) - 128), D the balanced digits of 2^(8b+32) mod p_j; then
-// sum_t 256^t C + bias_j = a_n 2^32 (mod p_j) is positive and below p 2^32,
-// and one REDC gives a_n mod p_j.
-//
-// MFMA 32x32x32 lane layout used here: A lane l holds row l%32, k = 16(l/32)
-// + 0..15; B lane l holds column l%32 and the same k; the result register r
-// of lane l is row 8(r/4) + 4(l/32) + r%4, column l%32.
+// int8 MFMA GEMM rows shared by the RNS and CRT kernels:
+//   acc[ct] += sum_ks A(rt, ks) x B(ct, ks)
+// A fragments come from a table in lane order and are prefetched eight
+// k-steps at a time, so one global latency covers eight MFMAs per column
+// tile. MFMA 32x32x32 lane layout: A lane l holds row l%32 and k = 16(l/32)
+// + 0..15; B lane l holds column l%32 and the same k; result register r of
+// lane l is row 8(r/4) + 4(l/32) + r%4, column l%32.
+// ---------------------------------------------------------------------------
+template <int NCT, class BF>
+__device__ __forceinline__ void gemm_rows(v16i (&acc)[NCT], const v4i* __restrict__ frag, int KS, BF bfrag) {
+    constexpr int PF = 8;
+    for (int ks0 = 0; ks0 < KS; ks0 += PF) {
+        v4i Af[PF];
+#pragma unroll
+        for (int i = 0; i < PF; ++i) Af[i] = ks0 + i < KS ? frag[(size_t)(ks0 + i) * 64] : (v4i){0, 0, 0, 0};
+#pragma unroll
+        for (int i = 0; i < PF; ++i) {
+            if (ks0 + i < KS) {
+#pragma unroll
+                for (int ct = 0; ct < NCT; ++ct) acc[ct] = mfma_i8(Af[i], bfrag(ct, ks0 + i), acc[ct]);
+            }
+        }
+    }
+}
+
+__device__ __forceinline__ long long slot_value(const v16i& acc, int g) {
+    return (long long)acc[4 * g] + (long long)acc[4 * g + 1] * 256 + (long long)acc[4 * g + 2] * 65536 +
+           (long long)acc[4 * g + 3] * 16777216;
+}
+
+// ---------------------------------------------------------------------------
+// RNS decomposition. Block = 8 waves and TN = 32 NCT consecutive coefficients,
+// copied verbatim (4L bytes each) into LDS. The GEMM is
+//   C[(j,t)][n] = sum_b D[(j,t)][b] (byte_b(a_n) - 128),
+// D the balanced digits of 2^(8b+32) mod p_j; sum_t 256^t C + bias_j is then
+// a_n 2^32 (mod p_j), positive and below p 2^32, and one REDC gives a_n mod p_j.
+// Columns of the last k-step past 4L read the next coefficient's bytes; the
+// matching digits of D are zero.
 // ---------------------------------------------------------------------------
 template <int NCT>
-__global__ void __launch_bounds__(256) k_rns(DevPlan D, const u32* __restrict__ a0, const u32* __restrict__ a1,
-                                              u32* __restrict__ R0, u32* __restrict__ R1, int stride) {
+__global__ void __launch_bounds__(512) k_rns(DevPlan D, const u32* __restrict__ a0, const u32* __restrict__ a1,
+                                              u32* __restrict__ R0, u32* __restrict__ R1, int vec) {
     extern __shared__ __align__(16) unsigned char lds8[];
+    u32* lds32 = reinterpret_cast<u32*>(lds8);
     constexpr int TN = 32 * NCT;
     const u32* a = blockIdx.y ? a1 : a0;
     u32* R = blockIdx.y ? R1 : R0;
@@ -306,36 +336,31 @@ __global__ void __launch_bounds__(256) k_rns(DevPlan D, const u32* __restrict__ 
     const u32 n0 = blockIdx.x * (u32)TN;
     const int cnt = (int)min((u32)TN, N - n0);
     const u32* src = a + (size_t)n0 * L;
-    const int kb = 32 * D.rns_ks;  // padded bytes per coefficient
+    const int words = TN * L;
 
-    for (int idx = (int)threadIdx.x; idx < TN * L; idx += (int)blockDim.x) {
-        const int c = idx / L, l = idx - c * L;
-        const u32 v = c < cnt ? src[idx] : 0u;
-        *reinterpret_cast<u32*>(lds8 + c * stride + 4 * l) = v;
+    if (vec && cnt == TN) {
+        const uint4* s4 = reinterpret_cast<const uint4*>(src);
+        uint4* d4 = reinterpret_cast<uint4*>(lds32);
+        for (int i = (int)threadIdx.x; i < words / 4; i += (int)blockDim.x) d4[i] = s4[i];
+    } else {
+        for (int i = (int)threadIdx.x; i < words; i += (int)blockDim.x) lds32[i] = i < cnt * L ? src[i] : 0u;
     }
-    const int padw = (kb - 4 * L) / 4;
-    for (int idx = (int)threadIdx.x; idx < TN * padw; idx += (int)blockDim.x) {
-        const int c = idx / padw, w = idx - c * padw;
-        *reinterpret_cast<u32*>(lds8 + c * stride + 4 * L + 4 * w) = 0x80808080u;
-    }
+    if (threadIdx.x < 8) lds32[words + threadIdx.x] = 0x80808080u;
     __syncthreads();
 
     const int lane = (int)threadIdx.x & 63, wave = (int)threadIdx.x >> 6, nw = (int)blockDim.x >> 6;
     const int half = lane >> 5, col = lane & 31;
-    const v4i flip = {(int)0x80808080u, (int)0x80808080u, (int)0x80808080u, (int)0x80808080u};
+    auto bfrag = [&](int ct, int ks) {
+        const u32* q = lds32 + (32 * ct + col) * L + 8 * ks + 4 * half;
+        const v4i B = {(int)(q[0] ^ 0x80808080u), (int)(q[1] ^ 0x80808080u), (int)(q[2] ^ 0x80808080u),
+                       (int)(q[3] ^ 0x80808080u)};
+        return B;
+    };
     for (int rt = wave; rt < D.rns_rt; rt += nw) {
         v16i acc[NCT];
 #pragma unroll
         for (int ct = 0; ct < NCT; ++ct) acc[ct] = (v16i){0};
-        const v4i* frag = D.rns_frag + (size_t)rt * D.rns_ks * 64 + lane;
-        for (int ks = 0; ks < D.rns_ks; ++ks) {
-            const v4i A = frag[(size_t)ks * 64];
-#pragma unroll
-            for (int ct = 0; ct < NCT; ++ct) {
-                const v4i B = *reinterpret_cast<const v4i*>(lds8 + (32 * ct + col) * stride + 32 * ks + 16 * half) ^ flip;
-                acc[ct] = mfma_i8(A, B, acc[ct]);
-            }
-        }
+        gemm_rows<NCT>(acc, D.rns_frag + (size_t)rt * D.rns_ks * 64 + lane, D.rns_ks, bfrag);
 #pragma unroll
         for (int g = 0; g < 4; ++g) {
             const int j = 8 * rt + 2 * g + half;
@@ -345,10 +370,7 @@ __global__ void __launch_bounds__(256) k_rns(DevPlan D, const u32* __restrict__ 
 #pragma unroll
             for (int ct = 0; ct < NCT; ++ct) {
                 const int c = 32 * ct + col;
-                if (c >= cnt) continue;
-                const long long s = (long long)acc[ct][4 * g] + (long long)acc[ct][4 * g + 1] * 256 +
-                                    (long long)acc[ct][4 * g + 2] * 65536 + (long long)acc[ct][4 * g + 3] * 16777216;
-                R[(size_t)j * N + n0 + c] = redc64((u64)(s + bias), p, pinv);
+                if (c < cnt) R[(size_t)j * N + n0 + c] = redc64((u64)(slot_value(acc[ct], g) + bias), p, pinv);
             }
         }
     }
@@ -377,49 +399,60 @@ __global__ void __launch_bounds__(256) k_crt(DevPlan D, const u32* __restrict__ 
     const u32 n0 = blockIdx.x * (u32)TN;
     const int cnt = (int)min((u32)TN, N - n0);
 
-    const int kp = 8 * D.crt_ks;
-    for (int idx = (int)threadIdx.x; idx < kp * TN; idx += (int)blockDim.x) {
-        const int j = idx / TN, c = idx - j * TN;
-        u32 y = 0;
-        if (j < K && c < cnt) {
-            const uint2 g = G[j];
-            y = shoup_mul(R[(size_t)j * N + n0 + c], g.x, g.y, D.p[j]);
+    // y bytes: thread (c, r) covers primes j = r, r + JS, ... for coefficient c,
+    // eight loads in flight per batch (lanes sweep c, so loads coalesce).
+    {
+        const int kp = 8 * D.crt_ks;
+        constexpr int JS = 256 / TN;
+        const int c = (int)threadIdx.x % TN, r = (int)threadIdx.x / TN;
+        const bool live = c < cnt;
+        const u32* Rc = R + n0 + c;
+        for (int j0 = r; j0 < kp; j0 += 8 * JS) {
+            u32 v[8];
+#pragma unroll
+            for (int i = 0; i < 8; ++i) {
+                const int j = j0 + i * JS;
+                v[i] = (live && j < K) ? Rc[(size_t)j * N] : 0u;
+            }
+#pragma unroll
+            for (int i = 0; i < 8; ++i) {
+                const int j = j0 + i * JS;
+                if (j < kp) {
+                    u32 y = 0;
+                    if (live && j < K) {
+                        const uint2 g = G[j];
+                        y = shoup_mul(v[i], g.x, g.y, D.p[j]);
+                    }
+                    *reinterpret_cast<u32*>(Y + c * ys + 4 * j) = y ^ 0x80808080u;
+                }
+            }
         }
-        *reinterpret_cast<u32*>(Y + c * ys + 4 * j) = y ^ 0x80808080u;
     }
     __syncthreads();
 
     const int lane = (int)threadIdx.x & 63, wave = (int)threadIdx.x >> 6, nw = (int)blockDim.x >> 6;
     const int half = lane >> 5, col = lane & 31;
+    auto bfrag = [&](int ct, int ks) {
+        return *reinterpret_cast<const v4i*>(Y + (32 * ct + col) * ys + 32 * ks + 16 * half);
+    };
     for (int rt = wave; rt < D.crt_rt; rt += nw) {
         v16i acc[NCT];
 #pragma unroll
         for (int ct = 0; ct < NCT; ++ct) acc[ct] = (v16i){0};
-        const v4i* frag = D.crt_frag + (size_t)rt * D.crt_ks * 64 + lane;
-        for (int ks = 0; ks < D.crt_ks; ++ks) {
-            const v4i A = frag[(size_t)ks * 64];
-#pragma unroll
-            for (int ct = 0; ct < NCT; ++ct) {
-                const v4i B = *reinterpret_cast<const v4i*>(Y + (32 * ct + col) * ys + 32 * ks + 16 * half);
-                acc[ct] = mfma_i8(A, B, acc[ct]);
-            }
-        }
+        gemm_rows<NCT>(acc, D.crt_frag + (size_t)rt * D.crt_ks * 64 + lane, D.crt_ks, bfrag);
 #pragma unroll
         for (int g = 0; g < 4; ++g) {
             const int slot = 8 * rt + 2 * g + half;
             if (slot >= nslot) continue;
 #pragma unroll
-            for (int ct = 0; ct < NCT; ++ct) {
-                const long long s = (long long)acc[ct][4 * g] + (long long)acc[ct][4 * g + 1] * 256 +
-                                    (long long)acc[ct][4 * g + 2] * 65536 + (long long)acc[ct][4 * g + 3] * 16777216;
-                Pp[(size_t)(32 * ct + col) * nslot + slot] = s;
-            }
+            for (int ct = 0; ct < NCT; ++ct) Pp[(size_t)(32 * ct + col) * nslot + slot] = slot_value(acc[ct], g);
         }
     }
     __syncthreads();
 
     u32* Z = reinterpret_cast<u32*>(Y);  // [TN][ZL], reuses the y bytes
     const int W = D.W, wl = W >> 5, wb = W & 31;
+    const u32* C0 = D.C0;
     for (int c = (int)threadIdx.x; c < cnt; c += (int)blockDim.x) {
         const long long* P = Pp + (size_t)c * nslot;
         u32* z = Z + (size_t)c * ZL;
@@ -429,12 +462,17 @@ __global__ void __launch_bounds__(256) k_crt(DevPlan D, const u32* __restrict__ 
         const u32 v = min((u32)((F + (1ull << 47)) >> 48), (u32)K);
         const u32* Tv = D.T + (size_t)v * L;
         long long carry = 0;
-        for (int i = 0; i < ZL; ++i) {
-            long long s = carry + (long long)D.C0[i];
-            if (i <= L) s += P[i];
-            if (i < L) s += Tv[i];
+#pragma unroll 8
+        for (int i = 0; i < L; ++i) {
+            const long long s = carry + P[i] + (long long)C0[i] + (long long)Tv[i];
             z[i] = (u32)s;
             carry = s >> 32;  // arithmetic: the running sum may be negative
+        }
+        {
+            long long s = carry + P[L] + (long long)C0[L];
+            z[L] = (u32)s;
+            s = (s >> 32) + (long long)C0[L + 1];
+            z[L + 1] = (u32)s;
         }
         for (;;) {
             u64 hi;
@@ -477,8 +515,8 @@ __global__ void __launch_bounds__(256) k_crt(DevPlan D, const u32* __restrict__ 
     __syncthreads();
     u32* dst = out + (size_t)n0 * L;
     for (int idx = (int)threadIdx.x; idx < cnt * L; idx += (int)blockDim.x) {
-        const int c = idx / L, l = idx - c * L;
-        dst[idx] = Z[(size_t)c * ZL + l];
+        const int c = L == 1 ? idx : (int)__umulhi((u32)idx, D.magic_L);  // idx / L
+        dst[idx] = Z[(size_t)c * ZL + (idx - c * L)];
     }
 }
 
@@ -518,10 +556,10 @@ int line_threads(int nb) { return nb >= 4 ? 1 << (nb - 4) : 1; }
 LaunchCfg choose_launch(const DevPlan& D, size_t max_lds) {
     LaunchCfg C;
     const u32 nrows = 1u << D.w1, ncols = 1u << D.w2;
-    C.rns_stride = 32 * D.rns_ks + 16;
+    C.rns_stride = 4 * D.L;
     C.rns_tn = 128;
-    while (C.rns_tn > 32 && (size_t)C.rns_tn * C.rns_stride > max_lds) C.rns_tn /= 2;
-    C.rns_lds = (size_t)C.rns_tn * C.rns_stride;
+    while (C.rns_tn > 32 && (size_t)C.rns_tn * C.rns_stride + 32 > max_lds) C.rns_tn /= 2;
+    C.rns_lds = (size_t)C.rns_tn * C.rns_stride + 32;
     C.crt_ys = 32 * D.crt_ks + 16;
     C.crt_tn = 128;
     auto crt_need = [&](int tn) { return (size_t)tn * C.crt_ys + (size_t)tn * (D.L + 3) * 8; };
@@ -534,11 +572,12 @@ LaunchCfg choose_launch(const DevPlan& D, size_t max_lds) {
 
 hipError_t launch_rns(const DevPlan& D, const LaunchCfg& C, const u32* a0, const u32* a1, u32* R0, u32* R1, int nops,
                       hipStream_t s) {
-    const dim3 grid((D.N + C.rns_tn - 1) / C.rns_tn, nops), block(256);
+    const dim3 grid((D.N + C.rns_tn - 1) / C.rns_tn, nops), block(512);
+    const int vec = ((uintptr_t)a0 % 16 == 0) && (nops < 2 || (uintptr_t)a1 % 16 == 0);
     switch (C.rns_tn) {
-        case 128: hipLaunchKernelGGL((k_rns<4>), grid, block, C.rns_lds, s, D, a0, a1, R0, R1, C.rns_stride); break;
-        case 64: hipLaunchKernelGGL((k_rns<2>), grid, block, C.rns_lds, s, D, a0, a1, R0, R1, C.rns_stride); break;
-        case 32: hipLaunchKernelGGL((k_rns<1>), grid, block, C.rns_lds, s, D, a0, a1, R0, R1, C.rns_stride); break;
+        case 128: hipLaunchKernelGGL((k_rns<4>), grid, block, C.rns_lds, s, D, a0, a1, R0, R1, vec); break;
+        case 64: hipLaunchKernelGGL((k_rns<2>), grid, block, C.rns_lds, s, D, a0, a1, R0, R1, vec); break;
+        case 32: hipLaunchKernelGGL((k_rns<1>), grid, block, C.rns_lds, s, D, a0, a1, R0, R1, vec); break;
         default: return hipErrorInvalidValue;
     }
     return hipGetLastError();
