@@ -77,6 +77,13 @@ int pm_sync(pm_state* s);
  * column forward, ms[3] fused rows, ms[4] column inverse, ms[5] CRT. */
 int pm_time_device(pm_state* s, const uint32_t* d_a, const uint32_t* d_b, uint32_t* d_c, int iters, float* ms);
 
+/* PCIe and host-copy rates (GB/s, best of `reps`) for transfers of `bytes`:
+ * out[0] pinned H2D, out[1] pinned D2H, out[2] pageable H2D, out[3] pageable
+ * D2H, out[4] host staging copy on 1 thread, out[5] on `threads` threads,
+ * out[6] H2D + D2H of bytes/2 each concurrently. Diagnostic; the reference's
+ * counterpart is its PM_DIAG setup benchmark (solve.cu:1245-1358). */
+int pm_probe_transfers(size_t bytes, int threads, int reps, double* out);
+
 /* ---- stage entry points, for stage-by-stage parity tests ----
  * All on device buffers, synchronous.
  *   rns:     R[K][N] = a mod p_j

@@ -263,6 +263,10 @@ int pm_time_device(pm_state* h, const uint32_t* d_a, const uint32_t* d_b, uint32
     });
 }
 
+int pm_probe_transfers(size_t bytes, int threads, int reps, double* out) {
+    return guarded([&] { pm::probe_transfers(bytes, threads, reps, out); });
+}
+
 int pm_stage_rns(pm_state* h, const uint32_t* d_a, uint32_t* d_R) {
     return guarded([&] {
         pm::State& S = *h->s;

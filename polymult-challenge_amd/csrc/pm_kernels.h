@@ -48,9 +48,11 @@ enum RowMode { ROW_FUSED = 0, ROW_FWD = 1, ROW_INV = 2 };
 // Chooses block shapes for a plan under the device's LDS limit.
 LaunchCfg choose_launch(const DevPlan& D, size_t max_lds);
 
-// R_i[j][n] = a_i[n] mod p_j for nops (1 or 2) operands.
+// R_i[j][n] = a_i[n] mod p_j for nops (1 or 2) operands, for the coefficients
+// n in [first, first + count) (count 0: to the end). a_i and R_i are the
+// whole arrays; `first` must be a multiple of 128.
 hipError_t launch_rns(const DevPlan& D, const LaunchCfg& C, const uint32_t* a0, const uint32_t* a1,
-                      uint32_t* R0, uint32_t* R1, int nops, hipStream_t s);
+                      uint32_t* R0, uint32_t* R1, int nops, hipStream_t s, uint32_t first = 0, uint32_t count = 0);
 // Column pass (stages 0 .. w1-1, or their inverse) on nops residue arrays.
 hipError_t launch_col(const DevPlan& D, const LaunchCfg& C, uint32_t* R0, uint32_t* R1, int nops, bool inverse,
                       hipStream_t s);
@@ -58,8 +60,8 @@ hipError_t launch_col(const DevPlan& D, const LaunchCfg& C, uint32_t* R0, uint32
 // the product carries 2^-32). ROW_FWD / ROW_INV: RB only.
 hipError_t launch_row(const DevPlan& D, const LaunchCfg& C, uint32_t* RB, const uint32_t* RA, RowMode mode,
                       hipStream_t s);
-// out[n][L] = CRT of R[j][n] (scaled by G) mod q.
+// out[n][L] = CRT of R[j][n] (scaled by G) mod q, for n in [first, first + count).
 hipError_t launch_crt(const DevPlan& D, const LaunchCfg& C, const uint32_t* R, uint32_t* out, const uint2* G,
-                      hipStream_t s);
+                      hipStream_t s, uint32_t first = 0, uint32_t count = 0);
 
 }  // namespace pm

@@ -325,7 +325,7 @@ __device__ __forceinline__ long long slot_value(const v16i& acc, int g) {
 // ---------------------------------------------------------------------------
 template <int NCT>
 __global__ void __launch_bounds__(512) k_rns(DevPlan D, const u32* __restrict__ a0, const u32* __restrict__ a1,
-                                              u32* __restrict__ R0, u32* __restrict__ R1, int vec) {
+                                              u32* __restrict__ R0, u32* __restrict__ R1, u32 count, int vec) {
     extern __shared__ __align__(16) unsigned char lds8[];
     u32* lds32 = reinterpret_cast<u32*>(lds8);
     constexpr int TN = 32 * NCT;
@@ -334,7 +334,7 @@ __global__ void __launch_bounds__(512) k_rns(DevPlan D, const u32* __restrict__ 
     const int L = D.L, K = D.K;
     const u32 N = D.N;
     const u32 n0 = blockIdx.x * (u32)TN;
-    const int cnt = (int)min((u32)TN, N - n0);
+    const int cnt = (int)min((u32)TN, count - n0);
     const u32* src = a + (size_t)n0 * L;
     const int words = TN * L;
 
@@ -389,7 +389,7 @@ __global__ void __launch_bounds__(512) k_rns(DevPlan D, const u32* __restrict__ 
 // ---------------------------------------------------------------------------
 template <int NCT>
 __global__ void __launch_bounds__(256) k_crt(DevPlan D, const u32* __restrict__ R, u32* __restrict__ out,
-                                              const uint2* __restrict__ G, int ys) {
+                                              const uint2* __restrict__ G, u32 count, int ys) {
     extern __shared__ __align__(16) unsigned char lds8[];
     constexpr int TN = 32 * NCT;
     const int L = D.L, K = D.K, nslot = L + 3, ZL = L + 2;
@@ -397,7 +397,7 @@ __global__ void __launch_bounds__(256) k_crt(DevPlan D, const u32* __restrict__ 
     unsigned char* Y = lds8;                                           // [TN][ys]
     long long* Pp = reinterpret_cast<long long*>(lds8 + (size_t)TN * ys);  // [TN][nslot]
     const u32 n0 = blockIdx.x * (u32)TN;
-    const int cnt = (int)min((u32)TN, N - n0);
+    const int cnt = (int)min((u32)TN, count - n0);
 
     // y bytes: thread (c, r) covers primes j = r, r + JS, ... for coefficient c,
     // eight loads in flight per batch (lanes sweep c, so loads coalesce).
@@ -571,13 +571,22 @@ LaunchCfg choose_launch(const DevPlan& D, size_t max_lds) {
 }
 
 hipError_t launch_rns(const DevPlan& D, const LaunchCfg& C, const u32* a0, const u32* a1, u32* R0, u32* R1, int nops,
-                      hipStream_t s) {
-    const dim3 grid((D.N + C.rns_tn - 1) / C.rns_tn, nops), block(512);
+                      hipStream_t s, u32 first, u32 count) {
+    if (count == 0) count = D.N - first;
+    if (first + count > D.N) return hipErrorInvalidValue;
+    const dim3 grid((count + C.rns_tn - 1) / C.rns_tn, nops), block(512);
+    const size_t L = (size_t)D.L;
+    a0 += first * L;
+    R0 += first;
+    if (nops > 1) {
+        a1 += first * L;
+        R1 += first;
+    }
     const int vec = ((uintptr_t)a0 % 16 == 0) && (nops < 2 || (uintptr_t)a1 % 16 == 0);
     switch (C.rns_tn) {
-        case 128: hipLaunchKernelGGL((k_rns<4>), grid, block, C.rns_lds, s, D, a0, a1, R0, R1, vec); break;
-        case 64: hipLaunchKernelGGL((k_rns<2>), grid, block, C.rns_lds, s, D, a0, a1, R0, R1, vec); break;
-        case 32: hipLaunchKernelGGL((k_rns<1>), grid, block, C.rns_lds, s, D, a0, a1, R0, R1, vec); break;
+        case 128: hipLaunchKernelGGL((k_rns<4>), grid, block, C.rns_lds, s, D, a0, a1, R0, R1, count, vec); break;
+        case 64: hipLaunchKernelGGL((k_rns<2>), grid, block, C.rns_lds, s, D, a0, a1, R0, R1, count, vec); break;
+        case 32: hipLaunchKernelGGL((k_rns<1>), grid, block, C.rns_lds, s, D, a0, a1, R0, R1, count, vec); break;
         default: return hipErrorInvalidValue;
     }
     return hipGetLastError();
@@ -607,12 +616,17 @@ hipError_t launch_row(const DevPlan& D, const LaunchCfg& C, u32* RB, const u32* 
     return hipErrorInvalidValue;
 }
 
-hipError_t launch_crt(const DevPlan& D, const LaunchCfg& C, const u32* R, u32* out, const uint2* G, hipStream_t s) {
-    const dim3 grid((D.N + C.crt_tn - 1) / C.crt_tn), block(256);
+hipError_t launch_crt(const DevPlan& D, const LaunchCfg& C, const u32* R, u32* out, const uint2* G, hipStream_t s,
+                      u32 first, u32 count) {
+    if (count == 0) count = D.N - first;
+    if (first + count > D.N) return hipErrorInvalidValue;
+    const dim3 grid((count + C.crt_tn - 1) / C.crt_tn), block(256);
+    R += first;
+    out += (size_t)first * D.L;
     switch (C.crt_tn) {
-        case 128: hipLaunchKernelGGL((k_crt<4>), grid, block, C.crt_lds, s, D, R, out, G, C.crt_ys); break;
-        case 64: hipLaunchKernelGGL((k_crt<2>), grid, block, C.crt_lds, s, D, R, out, G, C.crt_ys); break;
-        case 32: hipLaunchKernelGGL((k_crt<1>), grid, block, C.crt_lds, s, D, R, out, G, C.crt_ys); break;
+        case 128: hipLaunchKernelGGL((k_crt<4>), grid, block, C.crt_lds, s, D, R, out, G, count, C.crt_ys); break;
+        case 64: hipLaunchKernelGGL((k_crt<2>), grid, block, C.crt_lds, s, D, R, out, G, count, C.crt_ys); break;
+        case 32: hipLaunchKernelGGL((k_crt<1>), grid, block, C.crt_lds, s, D, R, out, G, count, C.crt_ys); break;
         default: return hipErrorInvalidValue;
     }
     return hipGetLastError();

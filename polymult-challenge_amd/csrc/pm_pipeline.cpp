@@ -1,30 +1,497 @@
-// Host-buffer multiply. First form: whole-operand copies on the compute
-// stream around the device-resident multiply.
+// Host-buffer multiply through PCIe: the drop-in path behind fherma_run.
+//
+// The call moves 2 N L 4 bytes in and N L 4 bytes out, so everything else is
+// arranged to hide behind those transfers:
+//   * a and b are cut into contiguous coefficient chunks. A host worker team
+//     copies chunk i from the caller's pageable buffer into pinned staging
+//     (non-temporal stores) while the copy engine moves chunk i-1 to HBM and
+//     the RNS kernel decomposes chunk i-2 on the compute stream;
+//   * once every chunk is resident the three NTT passes run;
+//   * the result leaves in chunks: CRT of chunk i on the compute stream, its
+//     device-to-host copy on a second copy stream, and the host team scatters
+//     chunk i-1 from pinned staging into the caller's buffer.
+// Chunks are contiguous coefficient ranges (the input layout is [N][L]), so
+// staging needs no gather; the column NTT pass simply starts after the last
+// input chunk instead of per chunk.
+//
+// PM_PROFILE=1 prints one phase line per call on stderr.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
+#include <atomic>
+#include <chrono>
+#include <condition_variable>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <functional>
+#include <mutex>
 #include <stdexcept>
 #include <string>
+#include <thread>
+#include <vector>
+
+#if defined(__x86_64__)
+#include <immintrin.h>
+#endif
+#if defined(__linux__)
+#include <sched.h>
+#endif
 
 #include "pm_state.h"
 
 namespace pm {
 
 namespace {
+
 void hip_check(hipError_t e, const char* what) {
     if (e != hipSuccess) throw std::runtime_error(std::string(what) + ": " + hipGetErrorString(e));
 }
+
+inline void relax() {
+#if defined(__x86_64__)
+    _mm_pause();
+#endif
+}
+
+// Copy that streams the destination past the caches (the staging buffers are
+// written once and read by DMA; the result is read later by the caller), so a
+// copy costs one read and one write of memory rather than three.
+#if defined(__x86_64__)
+__attribute__((target("avx2"))) void copy_stream_avx2(unsigned char* d, const unsigned char* s, size_t n) {
+    while (n && ((uintptr_t)d & 31)) {
+        *d++ = *s++;
+        --n;
+    }
+    size_t i = 0;
+    for (; i + 128 <= n; i += 128) {
+        const __m256i x0 = _mm256_loadu_si256((const __m256i*)(s + i));
+        const __m256i x1 = _mm256_loadu_si256((const __m256i*)(s + i + 32));
+        const __m256i x2 = _mm256_loadu_si256((const __m256i*)(s + i + 64));
+        const __m256i x3 = _mm256_loadu_si256((const __m256i*)(s + i + 96));
+        _mm256_stream_si256((__m256i*)(d + i), x0);
+        _mm256_stream_si256((__m256i*)(d + i + 32), x1);
+        _mm256_stream_si256((__m256i*)(d + i + 64), x2);
+        _mm256_stream_si256((__m256i*)(d + i + 96), x3);
+    }
+    if (i < n) std::memcpy(d + i, s + i, n - i);
+    _mm_sfence();
+}
+#endif
+
+void copy_stream(void* dst, const void* src, size_t n) {
+#if defined(__x86_64__)
+    static const bool avx2 = __builtin_cpu_supports("avx2");
+    if (avx2) {
+        copy_stream_avx2(static_cast<unsigned char*>(dst), static_cast<const unsigned char*>(src), n);
+        return;
+    }
+#endif
+    std::memcpy(dst, src, n);
+}
+
+int host_cpus() {
+#if defined(__linux__)
+    cpu_set_t set;
+    if (sched_getaffinity(0, sizeof(set), &set) == 0) return CPU_COUNT(&set);
+#endif
+    return (int)std::max(1u, std::thread::hardware_concurrency());
+}
+
 }  // namespace
 
-void State::pipeline_init() {}
-void State::pipeline_free() {}
+// A team of host threads that runs fn(0..n-1) with the calling thread taking
+// part. Between calls the workers sleep; while a multiply is in progress
+// (hot()) they spin so a copy job starts within microseconds.
+class Team {
+  public:
+    explicit Team(int workers) {
+        for (int i = 0; i < workers; ++i) threads_.emplace_back([this] { loop(); });
+    }
+    ~Team() {
+        {
+            std::lock_guard<std::mutex> lk(m_);
+            stop_ = true;
+        }
+        cv_.notify_all();
+        for (auto& t : threads_) t.join();
+    }
+    int size() const { return (int)threads_.size() + 1; }
+    void hot(bool on) {
+        {
+            std::lock_guard<std::mutex> lk(m_);
+            hot_ = on;
+        }
+        if (on) cv_.notify_all();
+    }
+    void run(int n, const std::function<void(int)>& fn) {
+        Job job{&fn, n};
+        current_.store(&job);
+        epoch_.fetch_add(1);
+        if (sleeping_.load() > 0) {
+            // taking the lock orders the wake-up after any sleeper's predicate check
+            { std::lock_guard<std::mutex> lk(m_); }
+            cv_.notify_all();
+        }
+        work(job);
+        while (job.done.load() < n) relax();
+        current_.store(nullptr);
+        while (busy_.load() > 0) relax();  // nobody still holds &job
+    }
+
+  private:
+    struct Job {
+        const std::function<void(int)>* fn;
+        int total;
+        std::atomic<int> next{0};
+        std::atomic<int> done{0};
+    };
+    static void work(Job& j) {
+        for (int i; (i = j.next.fetch_add(1)) < j.total;) {
+            (*j.fn)(i);
+            j.done.fetch_add(1);
+        }
+    }
+    void loop() {
+        uint64_t seen = epoch_.load();
+        for (;;) {
+            int idle = 0;
+            while (epoch_.load() == seen) {
+                relax();
+                if (++idle > 4096) {
+                    std::unique_lock<std::mutex> lk(m_);
+                    if (stop_) return;
+                    if (!hot_ && epoch_.load() == seen) {
+                        sleeping_.fetch_add(1);
+                        cv_.wait(lk, [&] { return stop_ || hot_ || epoch_.load() != seen; });
+                        sleeping_.fetch_sub(1);
+                        if (stop_) return;
+                    }
+                    idle = 0;
+                }
+            }
+            seen = epoch_.load();
+            busy_.fetch_add(1);
+            if (Job* j = current_.load()) work(*j);
+            busy_.fetch_sub(1);
+        }
+    }
+    std::vector<std::thread> threads_;
+    std::atomic<Job*> current_{nullptr};
+    std::atomic<uint64_t> epoch_{0};
+    std::atomic<int> busy_{0}, sleeping_{0};
+    std::mutex m_;
+    std::condition_variable cv_;
+    bool hot_ = false, stop_ = false;
+};
+
+struct Pipeline {
+    Team* team = nullptr;
+    hipStream_t s_in = nullptr, s_out = nullptr;
+    uint32_t* h_in[2] = {nullptr, nullptr};  // pinned staging, [N][L]
+    uint32_t* h_out = nullptr;
+    struct Chunk {
+        uint32_t first, count;
+    };
+    std::vector<Chunk> cin, cout;
+    std::vector<hipEvent_t> ev_in, ev_crt, ev_out;
+    bool profile = false;
+    // PM_PROFILE timeline: start of input, last H2D done, last RNS done,
+    // NTT passes done, first CRT done, first D2H done, last D2H done
+    hipEvent_t tl[3] = {nullptr, nullptr, nullptr};
+    hipEvent_t tl_more[4] = {nullptr, nullptr, nullptr, nullptr};
+};
+
+namespace {
+
+// Contiguous chunks: the body follows `sizes` (bytes; the last entry repeats),
+// then the `tail` sizes close the range; boundaries on multiples of `align`
+// coefficients. Small chunks at the ends start the copy engine early and
+// shorten the drain; large ones in the middle keep it near its large-transfer
+// rate (about 56 GB/s from 4 MB up, 40 GB/s at 1 MB).
+std::vector<Pipeline::Chunk> plan_chunks(uint32_t N, size_t bytes_per_coeff, const std::vector<size_t>& sizes,
+                                         const std::vector<size_t>& tail_sizes, uint32_t align) {
+    auto coeffs = [&](size_t b) {
+        const uint32_t c = (uint32_t)std::max<size_t>(align, b / bytes_per_coeff);
+        return (c + align - 1) / align * align;
+    };
+    std::vector<uint32_t> tail;
+    uint32_t tail_total = 0;
+    for (size_t b : tail_sizes) {
+        tail.push_back(coeffs(b));
+        tail_total += tail.back();
+    }
+    while (!tail.empty() && tail_total >= N) {
+        tail_total -= tail.front();
+        tail.erase(tail.begin());
+    }
+    const uint32_t body_end = N - tail_total;
+    std::vector<Pipeline::Chunk> out;
+    auto it = sizes.begin();
+    for (uint32_t f = 0; f < body_end;) {
+        const uint32_t take = std::min(coeffs(*it), body_end - f);
+        out.push_back({f, take});
+        f += take;
+        if (std::next(it) != sizes.end()) ++it;
+    }
+    uint32_t f = body_end;
+    for (uint32_t c : tail) {
+        out.push_back({f, c});
+        f += c;
+    }
+    return out;
+}
+
+// "0.5,2,8" (MiB) from the environment, or the default.
+std::vector<size_t> sizes_from_env(const char* name, std::vector<size_t> dflt) {
+    const char* e = std::getenv(name);
+    if (!e || !*e) return dflt;
+    std::vector<size_t> out;
+    for (const char* p = e; *p;) {
+        char* end = nullptr;
+        const double mb = std::strtod(p, &end);
+        if (end == p) break;
+        if (mb > 0) out.push_back((size_t)(mb * (1 << 20)));
+        p = *end == ',' ? end + 1 : end;
+    }
+    return out.empty() ? dflt : out;
+}
+
+void wait_event(hipEvent_t e) {
+    for (;;) {
+        const hipError_t r = hipEventQuery(e);
+        if (r == hipSuccess) return;
+        if (r != hipErrorNotReady) hip_check(r, "event");
+        relax();
+    }
+}
+
+}  // namespace
+
+void State::pipeline_init() {
+    Pipeline* P = new Pipeline;
+    pipe = P;
+    P->profile = std::getenv("PM_PROFILE") != nullptr;
+    int workers = std::min(15, std::max(1, host_cpus() - 1));
+    if (const char* e = std::getenv("PM_HOST_THREADS")) workers = std::max(0, std::atoi(e) - 1);
+    P->team = new Team(workers);
+    hip_check(hipStreamCreateWithFlags(&P->s_in, hipStreamNonBlocking), "stream");
+    hip_check(hipStreamCreateWithFlags(&P->s_out, hipStreamNonBlocking), "stream");
+    const size_t bytes = coeff_words * 4;
+    for (int i = 0; i < 2; ++i) hip_check(hipHostMalloc((void**)&P->h_in[i], bytes, hipHostMallocDefault), "pinned");
+    hip_check(hipHostMalloc((void**)&P->h_out, bytes, hipHostMallocDefault), "pinned");
+    const size_t bpc = (size_t)host.L * 4;
+    const size_t MiB = 1u << 20;
+    // Defaults from a sweep at N=65536/W=1747 (all within 40 us of each other;
+    // the copy engine, not the chunking, sets the pace).
+    P->cin = plan_chunks(host.N, bpc, sizes_from_env("PM_IN_CHUNKS", {MiB, 4 * MiB, 8 * MiB}),
+                         sizes_from_env("PM_IN_TAIL", {MiB}), 128);
+    P->cout = plan_chunks(host.N, bpc, sizes_from_env("PM_OUT_CHUNKS", {MiB, 14 * MiB}),
+                          sizes_from_env("PM_OUT_TAIL", {MiB / 2}), 128);
+    if (P->profile) {
+        for (int i = 0; i < 3; ++i) hip_check(hipEventCreate(&P->tl[i]), "event");
+        for (auto& e : P->tl_more) hip_check(hipEventCreate(&e), "event");
+    }
+    auto mk = [](std::vector<hipEvent_t>& v, size_t n) {
+        v.resize(n);
+        for (auto& e : v) hip_check(hipEventCreateWithFlags(&e, hipEventDisableTiming), "event");
+    };
+    mk(P->ev_in, 2 * P->cin.size());
+    mk(P->ev_crt, P->cout.size());
+    mk(P->ev_out, P->cout.size());
+    if (P->profile)
+        std::fprintf(stderr, "pm: N=%u W=%u L=%u K=%d, w1=%d w2=%d, chunks in %zu x2 / out %zu, %d host threads\n",
+                     host.N, host.W, host.L, host.K, host.w1, host.w2, P->cin.size(), P->cout.size(),
+                     P->team->size());
+}
+
+void State::pipeline_free() {
+    Pipeline* P = pipe;
+    if (!P) return;
+    if (P->s_in) (void)hipStreamSynchronize(P->s_in);
+    if (P->s_out) (void)hipStreamSynchronize(P->s_out);
+    for (auto e : P->ev_in) (void)hipEventDestroy(e);
+    for (auto e : P->ev_crt) (void)hipEventDestroy(e);
+    for (auto e : P->ev_out) (void)hipEventDestroy(e);
+    for (auto e : P->tl)
+        if (e) (void)hipEventDestroy(e);
+    for (auto e : P->tl_more)
+        if (e) (void)hipEventDestroy(e);
+    for (int i = 0; i < 2; ++i)
+        if (P->h_in[i]) (void)hipHostFree(P->h_in[i]);
+    if (P->h_out) (void)hipHostFree(P->h_out);
+    if (P->s_in) (void)hipStreamDestroy(P->s_in);
+    if (P->s_out) (void)hipStreamDestroy(P->s_out);
+    delete P->team;
+    delete P;
+    pipe = nullptr;
+}
 
 void State::run_host(const uint32_t* a, const uint32_t* b, uint32_t* c) {
-    const size_t bytes = coeff_words * 4;
-    hip_check(hipMemcpyAsync(d_in[0], a, bytes, hipMemcpyHostToDevice, stream), "H2D a");
-    hip_check(hipMemcpyAsync(d_in[1], b, bytes, hipMemcpyHostToDevice, stream), "H2D b");
-    hip_check(enqueue_multiply(d_in[0], d_in[1], d_out, stream), "multiply");
-    hip_check(hipMemcpyAsync(c, d_out, bytes, hipMemcpyDeviceToHost, stream), "D2H c");
-    hip_check(hipStreamSynchronize(stream), "synchronize");
+    Pipeline& P = *pipe;
+    using clock = std::chrono::steady_clock;
+    const auto t0 = clock::now();
+    const size_t L = host.L;
+    const int T = P.team->size();
+    const uint32_t* src[2] = {a, b};
+    struct Hot {
+        Team* t;
+        explicit Hot(Team* x) : t(x) { t->hot(true); }
+        ~Hot() { t->hot(false); }
+    } hot(P.team);
+
+    // Each copy job is split into T pieces of whole coefficients.
+    auto copy_job = [&](uint32_t* dst, const uint32_t* from, size_t words) {
+        const size_t piece = (words / L + T - 1) / T * L;
+        P.team->run(T, [&](int i) {
+            const size_t lo = std::min(words, (size_t)i * piece), hi = std::min(words, lo + piece);
+            if (hi > lo) copy_stream(dst + lo, from + lo, (hi - lo) * 4);
+        });
+    };
+
+    // ---- input: stage, H2D and RNS per chunk ----
+    const int nin = (int)P.cin.size();
+    if (P.profile) hip_check(hipEventRecord(P.tl[0], P.s_in), "record");
+    for (int op = 0; op < 2; ++op) {
+        for (int k = 0; k < nin; ++k) {
+            const auto ch = P.cin[k];
+            const size_t off = (size_t)ch.first * L, words = (size_t)ch.count * L;
+            copy_job(P.h_in[op] + off, src[op] + off, words);
+            const int i = op * nin + k;
+            hip_check(hipMemcpyAsync(d_in[op] + off, P.h_in[op] + off, words * 4, hipMemcpyHostToDevice, P.s_in),
+                      "H2D");
+            hip_check(hipEventRecord(P.ev_in[i], P.s_in), "record");
+            hip_check(hipStreamWaitEvent(stream, P.ev_in[i], 0), "wait");
+            hip_check(launch_rns(dev, cfg, d_in[op], nullptr, d_R[op], nullptr, 1, stream, ch.first, ch.count), "rns");
+        }
+    }
+    const auto t_in = clock::now();
+    if (P.profile) {
+        hip_check(hipEventRecord(P.tl[1], P.s_in), "record");
+        hip_check(hipEventRecord(P.tl[2], stream), "record");
+    }
+
+    // ---- the three NTT passes ----
+    hip_check(launch_col(dev, cfg, d_R[0], d_R[1], 2, false, stream), "col");
+    hip_check(launch_row(dev, cfg, d_R[1], d_R[0], ROW_FUSED, stream), "row");
+    hip_check(launch_col(dev, cfg, d_R[1], nullptr, 1, true, stream), "col inv");
+    if (P.profile) hip_check(hipEventRecord(P.tl_more[0], stream), "record");
+
+    // ---- output: CRT, then D2H per chunk ----
+    // The CRT of the first chunk runs alone so its copy starts early; the rest
+    // runs as one full-width launch (small CRT launches leave most CUs idle and
+    // would pace the copies). Each D2H waits for the launch that covers it.
+    const int nout = (int)P.cout.size();
+    const uint32_t first_n = P.cout[0].count;
+    hip_check(launch_crt(dev, cfg, d_R[1], d_out, dev.G, stream, 0, first_n), "crt");
+    hip_check(hipEventRecord(P.ev_crt[0], stream), "record");
+    if (P.profile) hip_check(hipEventRecord(P.tl_more[1], stream), "record");
+    if (nout > 1) {
+        hip_check(launch_crt(dev, cfg, d_R[1], d_out, dev.G, stream, first_n, host.N - first_n), "crt");
+        hip_check(hipEventRecord(P.ev_crt[1], stream), "record");
+    }
+    for (int k = 0; k < nout; ++k) {
+        const auto ch = P.cout[k];
+        const size_t off = (size_t)ch.first * L, words = (size_t)ch.count * L;
+        if (k < 2) hip_check(hipStreamWaitEvent(P.s_out, P.ev_crt[k], 0), "wait");
+        hip_check(hipMemcpyAsync(P.h_out + off, d_out + off, words * 4, hipMemcpyDeviceToHost, P.s_out), "D2H");
+        hip_check(hipEventRecord(P.ev_out[k], P.s_out), "record");
+        if (P.profile && k == 0) hip_check(hipEventRecord(P.tl_more[2], P.s_out), "record");
+    }
+    if (P.profile) hip_check(hipEventRecord(P.tl_more[3], P.s_out), "record");
+    clock::time_point t_first{};
+    for (int k = 0; k < nout; ++k) {
+        wait_event(P.ev_out[k]);
+        if (k == 0) t_first = clock::now();
+        const auto ch = P.cout[k];
+        const size_t off = (size_t)ch.first * L;
+        copy_job(c + off, P.h_out + off, (size_t)ch.count * L);
+    }
+    if (P.profile) {
+        auto us = [&](clock::time_point t) { return std::chrono::duration<double, std::micro>(t - t0).count(); };
+        std::fprintf(stderr, "pm: inputs staged+submitted %.1f us | first out chunk %.1f | done %.1f\n", us(t_in),
+                     us(t_first), us(clock::now()));
+        auto el = [&](hipEvent_t e) {
+            float ms = 0;
+            hip_check(hipEventElapsedTime(&ms, P.tl[0], e), "elapsed");
+            return 1000.0 * ms;
+        };
+        std::fprintf(stderr,
+                     "pm: gpu: last H2D %.1f us | last RNS %.1f | NTT passes %.1f | first CRT %.1f | first D2H %.1f | "
+                     "last D2H %.1f\n",
+                     el(P.tl[1]), el(P.tl[2]), el(P.tl_more[0]), el(P.tl_more[1]), el(P.tl_more[2]), el(P.tl_more[3]));
+    }
+}
+
+}  // namespace pm
+
+// ---------------------------------------------------------------------------
+// Transfer probe (the evidence behind the chunk sizes and thread counts):
+// PCIe copy rates from pinned and pageable memory and the host staging rate.
+// ---------------------------------------------------------------------------
+namespace pm {
+
+void probe_transfers(size_t bytes, int threads, int reps, double* out) {
+    using clock = std::chrono::steady_clock;
+    auto gbps = [&](double s) { return (double)bytes / s / 1e9; };
+    void *d = nullptr, *hp = nullptr, *hq = nullptr;
+    hip_check(hipMalloc(&d, bytes), "probe alloc");
+    hip_check(hipHostMalloc(&hp, bytes, hipHostMallocDefault), "probe pinned");
+    hip_check(hipHostMalloc(&hq, bytes, hipHostMallocDefault), "probe pinned");
+    std::vector<unsigned char> pg(bytes, 1), pg2(bytes, 0);
+    std::memset(hp, 2, bytes);
+    hipStream_t s, s2;
+    hip_check(hipStreamCreateWithFlags(&s, hipStreamNonBlocking), "probe stream");
+    hip_check(hipStreamCreateWithFlags(&s2, hipStreamNonBlocking), "probe stream");
+    auto time_it = [&](const std::function<void()>& f) {
+        f();
+        double best = 1e30;
+        for (int r = 0; r < reps; ++r) {
+            const auto t = clock::now();
+            f();
+            best = std::min(best, std::chrono::duration<double>(clock::now() - t).count());
+        }
+        return best;
+    };
+    // 0, 1: pinned H2D and D2H through the copy engine
+    out[0] = gbps(time_it([&] {
+        hip_check(hipMemcpyAsync(d, hp, bytes, hipMemcpyHostToDevice, s), "h2d");
+        hip_check(hipStreamSynchronize(s), "sync");
+    }));
+    out[1] = gbps(time_it([&] {
+        hip_check(hipMemcpyAsync(hq, d, bytes, hipMemcpyDeviceToHost, s), "d2h");
+        hip_check(hipStreamSynchronize(s), "sync");
+    }));
+    // 2, 3: pageable H2D and D2H (the runtime's own staging)
+    out[2] = gbps(time_it([&] { hip_check(hipMemcpy(d, pg.data(), bytes, hipMemcpyHostToDevice), "h2d pageable"); }));
+    out[3] = gbps(time_it([&] { hip_check(hipMemcpy(pg2.data(), d, bytes, hipMemcpyDeviceToHost), "d2h pageable"); }));
+    // 4, 5: host staging copy pageable -> pinned, one thread and `threads` threads
+    out[4] = gbps(time_it([&] { copy_stream(hp, pg.data(), bytes); }));
+    {
+        Team team(std::max(0, threads - 1));
+        team.hot(true);
+        const int T = team.size();
+        const size_t piece = (bytes / 64 + T - 1) / T * 64;
+        out[5] = gbps(time_it([&] {
+            team.run(T, [&](int i) {
+                const size_t lo = std::min(bytes, (size_t)i * piece), hi = std::min(bytes, lo + piece);
+                if (hi > lo) copy_stream((char*)hp + lo, pg.data() + lo, hi - lo);
+            });
+        }));
+        // 6: both copy directions at once (pinned), full duplex
+        out[6] = gbps(time_it([&] {
+            hip_check(hipMemcpyAsync(d, hp, bytes / 2, hipMemcpyHostToDevice, s), "h2d");
+            hip_check(hipMemcpyAsync(hq, (char*)d + bytes / 2, bytes / 2, hipMemcpyDeviceToHost, s2), "d2h");
+            hip_check(hipStreamSynchronize(s), "sync");
+            hip_check(hipStreamSynchronize(s2), "sync");
+        }));
+        team.hot(false);
+    }
+    (void)hipStreamDestroy(s);
+    (void)hipStreamDestroy(s2);
+    (void)hipFree(d);
+    (void)hipHostFree(hp);
+    (void)hipHostFree(hq);
 }
 
 }  // namespace pm

@@ -41,4 +41,10 @@ struct State {
     ~State();
 };
 
+// Measures, for `bytes` per transfer (best of `reps`), in GB/s:
+// out[0] pinned H2D, out[1] pinned D2H, out[2] pageable H2D, out[3] pageable
+// D2H, out[4] host staging copy on one thread, out[5] on `threads` threads,
+// out[6] H2D and D2H of bytes/2 each at once (aggregate).
+void probe_transfers(size_t bytes, int threads, int reps, double* out);
+
 }  // namespace pm

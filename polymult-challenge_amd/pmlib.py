@@ -22,7 +22,7 @@ LIB_PATH = HERE / "libpolymult.so"
 C_SYMBOLS = (
     "pm_init", "pm_run", "pm_run_device", "pm_free", "pm_last_error", "pm_query", "pm_point_info",
     "pm_dev_alloc", "pm_dev_free", "pm_copy_to_device", "pm_copy_to_host", "pm_sync", "pm_time_device",
-    "pm_stage_rns", "pm_stage_ntt_fwd", "pm_stage_ntt_inv", "pm_stage_crt",
+    "pm_stage_rns", "pm_stage_ntt_fwd", "pm_stage_ntt_inv", "pm_stage_crt", "pm_probe_transfers",
 )
 # The C++ entry points of include/fherma.h, mangled as the reference's harness binds them.
 CXX_SYMBOLS = (
@@ -74,6 +74,8 @@ def load() -> ctypes.CDLL:
         "pm_stage_ntt_fwd": ([_vp, _vp], ctypes.c_int),
         "pm_stage_ntt_inv": ([_vp, _vp], ctypes.c_int),
         "pm_stage_crt": ([_vp, _vp, _vp], ctypes.c_int),
+        "pm_probe_transfers": ([ctypes.c_size_t, ctypes.c_int, ctypes.c_int, ctypes.POINTER(ctypes.c_double)],
+                               ctypes.c_int),
     }
     for name, (args, res) in sig.items():
         f = getattr(lib, name)
@@ -106,6 +108,15 @@ def point_info(N: int, W: int, L: int, q: np.ndarray, cap: int = 1024):
     primes = np.zeros(cap, dtype=np.uint32)
     _check(lib.pm_point_info(N, W, L, _ptr(q), q.size, ctypes.byref(K), ctypes.byref(delta), _ptr(primes), cap))
     return K.value, delta.value, primes[:K.value].copy()
+
+
+def probe_transfers(nbytes: int, threads: int = 16, reps: int = 5) -> dict:
+    """PCIe and host staging rates in GB/s (see pm_probe_transfers)."""
+    out = (ctypes.c_double * 7)()
+    _check(load().pm_probe_transfers(nbytes, threads, reps, out))
+    keys = ["h2d_pinned", "d2h_pinned", "h2d_pageable", "d2h_pageable", "stage_1thread", "stage_threads",
+            "duplex_pinned"]
+    return {k: round(v, 2) for k, v in zip(keys, out)}
 
 
 class DeviceBuffer:
