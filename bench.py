@@ -85,6 +85,30 @@ def cpu_baseline(q: int, sample_cases: int) -> dict | None:
             "per_call_s": secs, "output_matches_oracle": ok}
 
 
+def batched(pmlib, O, count: int, points, seed: int, check: int, iters: int) -> dict:
+    """One persistent hipGraph launch over `points`; mults/s = products / launch time."""
+    prods, ops = [], []
+    for i, (n, w) in enumerate(points):
+        q = O.spec_q(n, w)
+        prods.append((n, w, q))
+        ops.append(O.operands("uniform", n, w, q, seed + i))
+    B = pmlib.Batch(prods)
+    da = [pmlib.DeviceBuffer.from_array(a) for a, _ in ops]
+    db = [pmlib.DeviceBuffer.from_array(b) for _, b in ops]
+    dc = [pmlib.DeviceBuffer(n * O.limbs_of(w) * 4) for n, w in points]
+    tuned = B.autotune(da, db, dc, 3)
+    B.run(da, db, dc)
+    B.sync()
+    exact = True
+    for i in range(min(check, count)):
+        n, w, q = prods[i]
+        exact &= bool(np.array_equal(dc[i].download((n, O.limbs_of(w))), O.OraclePlan(n, w, q).multiply(*ops[i])))
+    ms = B.time(da, db, dc, iters)
+    io = sum(12 * n * O.limbs_of(w) for n, w in points)
+    return {"products": count, "ms_per_launch": round(ms, 4), "mults_per_s": round(count / (ms * 1e-3), 1),
+            "io_bytes": io, "bit_exact_checked": min(check, count), "bit_exact": exact, "autotune": tuned}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -158,6 +182,14 @@ def main():
                   "pct_pcie_peak": round(100 * pcie_bytes / e2e_med / (PCIE_PEAK_GBS * 1e9), 1),
                   "bit_exact": bool(np.array_equal(c_host, c_dev))}
 
+    # batched: the same point 16 times, and the mixed batch of 64 uneven products
+    del m, da, db, dc
+    batch_same = batched(pmlib, O, 16, [(N, W)] * 16, SEED + 100, 2, 20)
+    rng = np.random.default_rng(SEED)
+    mix = [(int(2 ** rng.integers(11, 17)), int(rng.integers(128, 2049))) for _ in range(64)]
+    batch_mix = batched(pmlib, O, 64, mix, SEED + 200, 4, 20)
+    batch_mix["workload"] = "64 products, N log-uniform in 2^11..2^16, W uniform in [128, 2048], seed %d" % SEED
+
     cpu = None if args.no_cpu else cpu_baseline(q, args.cpu_cases)
     out = {
         "metric": METRIC, "value": round(value, 2), "unit": "mults/s", "n_gpus": 1, "steps": args.steps,
@@ -168,7 +200,8 @@ def main():
                    "operands": "mixed per coefficient: 1/2 uniform [0,q), 1/4 q-1, 1/4 < 2^32", "seed": SEED},
         "bit_exact": correct,
         "pct_hbm_peak": round(100 * pcie_bytes / (ms_step * 1e-3) / (HBM_PEAK_GBS * 1e9), 3),
-        "roofline": roofline, "kernels": per_kernel, "end_to_end": end_to_end, "cpu_baseline": cpu,
+        "roofline": roofline, "kernels": per_kernel, "end_to_end": end_to_end,
+        "batched_same_point": batch_same, "batched_mixed": batch_mix, "cpu_baseline": cpu,
     }
     print(json.dumps(out))
 

@@ -96,6 +96,32 @@ int pm_stage_ntt_fwd(pm_state* s, uint32_t* d_R);
 int pm_stage_ntt_inv(pm_state* s, uint32_t* d_R);
 int pm_stage_crt(pm_state* s, const uint32_t* d_X, uint32_t* d_c);
 
+/* ---- batched form: many products of uneven N and W in one persistent launch ----
+ * No counterpart in the reference (it multiplies one pair per call). Products
+ * share one RNS prime set (p = 1 mod 2^17) and one twiddle table; their K
+ * follows the same rule as pm_init. q_concat holds each product's q (L_i =
+ * ceil(W_i / 32) limbs) back to back. One batch runs at a time. */
+typedef struct pm_batch pm_batch;
+int pm_batch_create(uint32_t count, const uint32_t* N, const uint32_t* W, const uint32_t* q_concat, pm_batch** out);
+void pm_batch_free(pm_batch* b);
+const char* pm_batch_last_error(void);
+int pm_batch_product_info(const pm_batch* b, uint32_t i, pm_info* out);
+/* Enqueues every product (device buffers, N_i*L_i u32 each, 16-byte aligned)
+ * as one hipGraph launch on `stream` (NULL: the batch's own). */
+int pm_batch_run(pm_batch* b, const uint32_t* const* d_a, const uint32_t* const* d_b, uint32_t* const* d_c,
+                 void* stream);
+/* Waits for the device and reports a failed batch (a task wait that hit its
+ * bound). */
+int pm_batch_sync(pm_batch* b);
+/* Average milliseconds of one batch launch over `iters` graph replays. */
+int pm_batch_time(pm_batch* b, const uint32_t* const* d_a, const uint32_t* const* d_b, uint32_t* const* d_c,
+                  int iters, float* ms);
+/* Times every candidate (column tile 8/16/32, 4-step split of odd logN,
+ * 1 or 2 resident blocks per CU), keeps the fastest; best[0..3] = column
+ * tile, split, blocks per CU, microseconds per launch. */
+int pm_batch_autotune(pm_batch* b, const uint32_t* const* d_a, const uint32_t* const* d_b, uint32_t* const* d_c,
+                      int iters, int* best);
+
 #ifdef __cplusplus
 }
 #endif

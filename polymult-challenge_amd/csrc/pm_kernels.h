@@ -16,7 +16,8 @@ struct DevPlan {
     uint64_t delta;
     const uint32_t* p;     // [K]
     const uint32_t* pinv;  // [K]
-    const uint2* tw;       // [K][N] Shoup pairs of psi^bitrev(k)
+    const uint2* tw;       // [K][tw_stride] Shoup pairs of psi^bitrev(k), k < N
+    uint32_t tw_stride;    // N for a single point; 65536 for the shared batch table
     const v4i* rns_frag;   // [rns_rt][rns_ks][64]
     const uint64_t* rns_bias;
     int rns_rt, rns_ks;
@@ -63,5 +64,41 @@ hipError_t launch_row(const DevPlan& D, const LaunchCfg& C, uint32_t* RB, const 
 // out[n][L] = CRT of R[j][n] (scaled by G) mod q, for n in [first, first + count).
 hipError_t launch_crt(const DevPlan& D, const LaunchCfg& C, const uint32_t* R, uint32_t* out, const uint2* G,
                       hipStream_t s, uint32_t first = 0, uint32_t count = 0);
+
+// ---- batched products: one launch per phase over every product's tiles ----
+enum BatchPhase { PH_RNS = 0, PH_COLF = 1, PH_ROW = 2, PH_COLI = 3, PH_CRT = 4, PH_COUNT = 5 };
+
+// One product of a batch as the phase kernels see it.
+struct BatchProduct {
+    DevPlan D;
+    const uint32_t* a;
+    const uint32_t* b;
+    uint32_t* c;
+    uint32_t* R0;
+    uint32_t* R1;
+    int col_tc, row_rpb, crt_ys;
+};
+
+// A tile of one phase: product and tile coordinates
+//   RNS  x = first coefficient (128 per tile), y = operand (0 a, 1 b)
+//   COLF x = first column, y = prime | operand << 16
+//   ROW  x = first row, y = prime
+//   COLI x = first column, y = prime
+//   CRT  x = first coefficient (64 per tile)
+struct BatchTask {
+    uint32_t prod, x, y;
+};
+
+constexpr int kBatchThreads = 256;
+constexpr int kBatchRnsTile = 128;
+constexpr int kBatchCrtTile = 64;
+
+// Zeroes n words (the persistent tile counters) with a kernel, so the launch
+// can sit in a captured graph.
+hipError_t launch_batch_reset(uint32_t* ctl, uint32_t nctl, hipStream_t s);
+// Runs the ntasks tiles of one phase. ctr == nullptr: one block per tile;
+// otherwise `grid` persistent blocks claim tiles from *ctr (zeroed before).
+hipError_t launch_batch_phase(int phase, const BatchProduct* prods, const BatchTask* tasks, uint32_t ntasks,
+                              uint32_t* ctr, int grid, size_t lds, hipStream_t s);
 
 }  // namespace pm

@@ -127,78 +127,97 @@ __device__ __forceinline__ void inv_pass1(u32 (&x)[Line<NB>::E], const uint2* __
 }
 
 // ---------------------------------------------------------------------------
-// Column pass. Block = tc columns x T threads; LDS tile [2^NB][tc] for the
+// Column tile: columns [col0, col0 + tc) of prime j, tc T threads (threads
+// from tc T on only take part in the barriers). LDS tile [2^NB][tc] for the
 // transpose between the two register passes. Threads sweep columns fastest,
 // so every global access of a wave reads or writes whole row segments.
 // ---------------------------------------------------------------------------
 template <int NB, bool INV>
-__global__ void __launch_bounds__(1024) k_col(DevPlan D, u32* __restrict__ R0, u32* __restrict__ R1, int tc) {
+__device__ __forceinline__ void col_tile(const DevPlan& D, u32* __restrict__ R, int j, int col0, int tc, u32* lds,
+                                         int tid) {
     using Ln = Line<NB>;
     constexpr int E = Ln::E, T = Ln::T, R2 = Ln::R2;
-    extern __shared__ u32 lds[];
-    const int c = (int)threadIdx.x % tc, t = (int)threadIdx.x / tc;
-    u32* R = blockIdx.z ? R1 : R0;
-    const int j = (int)blockIdx.y;
+    const bool on = tid < tc * T;
+    const int c = tid % tc, t = tid / tc;
     const u32 p = D.p[j];
-    const uint2* tab = D.tw + (size_t)j * D.N;
+    const uint2* tab = D.tw + (size_t)j * D.tw_stride;
     const size_t ncols = (size_t)1 << D.w2;
-    u32* col = R + (size_t)j * D.N + (size_t)blockIdx.x * tc + c;
+    u32* col = R + (size_t)j * D.N + col0 + c;
     u32 x[1][E];
     if (!INV) {
+        if (on) {
 #pragma unroll
-        for (int k = 0; k < E; ++k) x[0][k] = col[(size_t)(t + k * T) * ncols];
-        fwd_pass1<NB, 1>(x, tab, 0, 0, p);
+            for (int k = 0; k < E; ++k) x[0][k] = col[(size_t)(t + k * T) * ncols];
+            fwd_pass1<NB, 1>(x, tab, 0, 0, p);
+        }
         if constexpr (R2 > 0) {
+            if (on) {
 #pragma unroll
-            for (int k = 0; k < E; ++k) lds[(t + k * T) * tc + c] = x[0][k];
+                for (int k = 0; k < E; ++k) lds[(t + k * T) * tc + c] = x[0][k];
+            }
             __syncthreads();
+            if (on) {
 #pragma unroll
-            for (int k = 0; k < E; ++k) x[0][k] = lds[(t * E + k) * tc + c];
-            fwd_pass2<NB, 1>(x, tab, 0, 0, t, p);
+                for (int k = 0; k < E; ++k) x[0][k] = lds[(t * E + k) * tc + c];
+                fwd_pass2<NB, 1>(x, tab, 0, 0, t, p);
 #pragma unroll
-            for (int k = 0; k < E; ++k) col[(size_t)(t * E + k) * ncols] = x[0][k];
-        } else {
+                for (int k = 0; k < E; ++k) col[(size_t)(t * E + k) * ncols] = x[0][k];
+            }
+        } else if (on) {
 #pragma unroll
             for (int k = 0; k < E; ++k) col[(size_t)(t + k * T) * ncols] = x[0][k];
         }
     } else {
         if constexpr (R2 > 0) {
+            if (on) {
 #pragma unroll
-            for (int k = 0; k < E; ++k) x[0][k] = col[(size_t)(t * E + k) * ncols];
-            inv_pass2<NB>(x[0], tab, 0, 0, t, p);
+                for (int k = 0; k < E; ++k) x[0][k] = col[(size_t)(t * E + k) * ncols];
+                inv_pass2<NB>(x[0], tab, 0, 0, t, p);
 #pragma unroll
-            for (int k = 0; k < E; ++k) lds[(t * E + k) * tc + c] = x[0][k];
+                for (int k = 0; k < E; ++k) lds[(t * E + k) * tc + c] = x[0][k];
+            }
             __syncthreads();
+            if (on) {
 #pragma unroll
-            for (int k = 0; k < E; ++k) x[0][k] = lds[(t + k * T) * tc + c];
-        } else {
+                for (int k = 0; k < E; ++k) x[0][k] = lds[(t + k * T) * tc + c];
+            }
+        } else if (on) {
 #pragma unroll
             for (int k = 0; k < E; ++k) x[0][k] = col[(size_t)(t + k * T) * ncols];
         }
-        inv_pass1<NB>(x[0], tab, 0, 0, p);
+        if (on) {
+            inv_pass1<NB>(x[0], tab, 0, 0, p);
 #pragma unroll
-        for (int k = 0; k < E; ++k) col[(size_t)(t + k * T) * ncols] = x[0][k];
+            for (int k = 0; k < E; ++k) col[(size_t)(t + k * T) * ncols] = x[0][k];
+        }
     }
 }
 
+template <int NB, bool INV>
+__global__ void __launch_bounds__(1024) k_col(DevPlan D, u32* __restrict__ R0, u32* __restrict__ R1, int tc) {
+    extern __shared__ u32 lds[];
+    col_tile<NB, INV>(D, blockIdx.z ? R1 : R0, (int)blockIdx.y, (int)blockIdx.x * tc, tc, lds, (int)threadIdx.x);
+}
+
 // ---------------------------------------------------------------------------
-// Row pass. Block = rpb rows x T threads. LDS rows are padded by 16 words and
-// XOR-swizzled inside each 16-word group so that both the strided pass-1
-// writes and the contiguous pass-2 reads spread over the banks.
+// Row tile: rows [row0, row0 + rpb) of prime j, rpb T threads. LDS rows are
+// padded by 16 words and XOR-swizzled inside each 16-word group so that both
+// the strided pass-1 writes and the contiguous pass-2 reads spread over the
+// banks. LDS: 2 rpb (2^NB + 16) words.
 // ---------------------------------------------------------------------------
 __device__ __forceinline__ int swz(int e) { return e ^ ((e >> 4) & 15); }
 
 template <int NB, int MODE>
-__global__ void __launch_bounds__(256) k_row(DevPlan D, u32* __restrict__ RB, const u32* __restrict__ RA, int rpb) {
+__device__ __forceinline__ void row_tile(const DevPlan& D, u32* __restrict__ RB, const u32* __restrict__ RA, int j,
+                                         u32 row0, int rpb, u32* lds, int tid) {
     using Ln = Line<NB>;
     constexpr int E = Ln::E, T = Ln::T, R2 = Ln::R2;
     constexpr int RS = (1 << NB) + (R2 > 0 ? 16 : 0);
-    extern __shared__ u32 lds[];
-    const int r = (int)threadIdx.x / T, t = (int)threadIdx.x % T;
-    const int j = (int)blockIdx.y;
-    const u32 row = blockIdx.x * (u32)rpb + (u32)r;
+    const bool on = tid < rpb * T;
+    const int r = on ? tid / T : 0, t = tid % T;
+    const u32 row = row0 + (u32)r;
     const u32 p = D.p[j];
-    const uint2* tab = D.tw + (size_t)j * D.N;
+    const uint2* tab = D.tw + (size_t)j * D.tw_stride;
     const size_t off = (size_t)j * D.N + ((size_t)row << NB);
     u32* rb = RB + off;
     const int s0 = D.w1;
@@ -209,78 +228,110 @@ __global__ void __launch_bounds__(256) k_row(DevPlan D, u32* __restrict__ RB, co
         const u32* ra = RA + off;
         const u32 pinv = D.pinv[j];
         u32 x[2][E];
-#pragma unroll
-        for (int k = 0; k < E; ++k) {
-            x[0][k] = ra[t + k * T];
-            x[1][k] = rb[t + k * T];
-        }
-        fwd_pass1<NB, 2>(x, tab, s0, row, p);
-        if constexpr (R2 > 0) {
+        if (on) {
 #pragma unroll
             for (int k = 0; k < E; ++k) {
-                la[swz(t + k * T)] = x[0][k];
-                lb[swz(t + k * T)] = x[1][k];
+                x[0][k] = ra[t + k * T];
+                x[1][k] = rb[t + k * T];
+            }
+            fwd_pass1<NB, 2>(x, tab, s0, row, p);
+        }
+        if constexpr (R2 > 0) {
+            if (on) {
+#pragma unroll
+                for (int k = 0; k < E; ++k) {
+                    la[swz(t + k * T)] = x[0][k];
+                    lb[swz(t + k * T)] = x[1][k];
+                }
             }
             __syncthreads();
+            if (on) {
 #pragma unroll
-            for (int k = 0; k < E; ++k) {
-                x[0][k] = la[swz(t * E + k)];
-                x[1][k] = lb[swz(t * E + k)];
+                for (int k = 0; k < E; ++k) {
+                    x[0][k] = la[swz(t * E + k)];
+                    x[1][k] = lb[swz(t * E + k)];
+                }
+                fwd_pass2<NB, 2>(x, tab, s0, row, t, p);
             }
-            fwd_pass2<NB, 2>(x, tab, s0, row, t, p);
         }
         u32 y[E];
+        if (on) {
 #pragma unroll
-        for (int k = 0; k < E; ++k) y[k] = mont_mul(x[0][k], x[1][k], p, pinv);
-        if constexpr (R2 > 0) {
-            inv_pass2<NB>(y, tab, s0, row, t, p);
-            __syncthreads();
-#pragma unroll
-            for (int k = 0; k < E; ++k) lb[swz(t * E + k)] = y[k];
-            __syncthreads();
-#pragma unroll
-            for (int k = 0; k < E; ++k) y[k] = lb[swz(t + k * T)];
+            for (int k = 0; k < E; ++k) y[k] = mont_mul(x[0][k], x[1][k], p, pinv);
         }
-        inv_pass1<NB>(y, tab, s0, row, p);
+        if constexpr (R2 > 0) {
+            if (on) inv_pass2<NB>(y, tab, s0, row, t, p);
+            __syncthreads();
+            if (on) {
 #pragma unroll
-        for (int k = 0; k < E; ++k) rb[t + k * T] = y[k];
+                for (int k = 0; k < E; ++k) lb[swz(t * E + k)] = y[k];
+            }
+            __syncthreads();
+            if (on) {
+#pragma unroll
+                for (int k = 0; k < E; ++k) y[k] = lb[swz(t + k * T)];
+            }
+        }
+        if (on) {
+            inv_pass1<NB>(y, tab, s0, row, p);
+#pragma unroll
+            for (int k = 0; k < E; ++k) rb[t + k * T] = y[k];
+        }
     } else if constexpr (MODE == ROW_FWD) {
         u32 x[1][E];
+        if (on) {
 #pragma unroll
-        for (int k = 0; k < E; ++k) x[0][k] = rb[t + k * T];
-        fwd_pass1<NB, 1>(x, tab, s0, row, p);
+            for (int k = 0; k < E; ++k) x[0][k] = rb[t + k * T];
+            fwd_pass1<NB, 1>(x, tab, s0, row, p);
+        }
         if constexpr (R2 > 0) {
+            if (on) {
 #pragma unroll
-            for (int k = 0; k < E; ++k) lb[swz(t + k * T)] = x[0][k];
+                for (int k = 0; k < E; ++k) lb[swz(t + k * T)] = x[0][k];
+            }
             __syncthreads();
+            if (on) {
 #pragma unroll
-            for (int k = 0; k < E; ++k) x[0][k] = lb[swz(t * E + k)];
-            fwd_pass2<NB, 1>(x, tab, s0, row, t, p);
+                for (int k = 0; k < E; ++k) x[0][k] = lb[swz(t * E + k)];
+                fwd_pass2<NB, 1>(x, tab, s0, row, t, p);
 #pragma unroll
-            for (int k = 0; k < E; ++k) rb[t * E + k] = x[0][k];
-        } else {
+                for (int k = 0; k < E; ++k) rb[t * E + k] = x[0][k];
+            }
+        } else if (on) {
 #pragma unroll
             for (int k = 0; k < E; ++k) rb[t + k * T] = x[0][k];
         }
     } else {  // ROW_INV
         u32 y[E];
         if constexpr (R2 > 0) {
+            if (on) {
 #pragma unroll
-            for (int k = 0; k < E; ++k) y[k] = rb[t * E + k];
-            inv_pass2<NB>(y, tab, s0, row, t, p);
+                for (int k = 0; k < E; ++k) y[k] = rb[t * E + k];
+                inv_pass2<NB>(y, tab, s0, row, t, p);
 #pragma unroll
-            for (int k = 0; k < E; ++k) lb[swz(t * E + k)] = y[k];
+                for (int k = 0; k < E; ++k) lb[swz(t * E + k)] = y[k];
+            }
             __syncthreads();
+            if (on) {
 #pragma unroll
-            for (int k = 0; k < E; ++k) y[k] = lb[swz(t + k * T)];
-        } else {
+                for (int k = 0; k < E; ++k) y[k] = lb[swz(t + k * T)];
+            }
+        } else if (on) {
 #pragma unroll
             for (int k = 0; k < E; ++k) y[k] = rb[t + k * T];
         }
-        inv_pass1<NB>(y, tab, s0, row, p);
+        if (on) {
+            inv_pass1<NB>(y, tab, s0, row, p);
 #pragma unroll
-        for (int k = 0; k < E; ++k) rb[t + k * T] = y[k];
+            for (int k = 0; k < E; ++k) rb[t + k * T] = y[k];
+        }
     }
+}
+
+template <int NB, int MODE>
+__global__ void __launch_bounds__(256) k_row(DevPlan D, u32* __restrict__ RB, const u32* __restrict__ RA, int rpb) {
+    extern __shared__ u32 lds[];
+    row_tile<NB, MODE>(D, RB, RA, (int)blockIdx.y, blockIdx.x * (u32)rpb, rpb, lds, (int)threadIdx.x);
 }
 
 // ---------------------------------------------------------------------------
@@ -324,16 +375,11 @@ __device__ __forceinline__ long long slot_value(const v16i& acc, int g) {
 // matching digits of D are zero.
 // ---------------------------------------------------------------------------
 template <int NCT>
-__global__ void __launch_bounds__(512) k_rns(DevPlan D, const u32* __restrict__ a0, const u32* __restrict__ a1,
-                                              u32* __restrict__ R0, u32* __restrict__ R1, u32 count, int vec) {
-    extern __shared__ __align__(16) unsigned char lds8[];
-    u32* lds32 = reinterpret_cast<u32*>(lds8);
+__device__ __forceinline__ void rns_tile(const DevPlan& D, const u32* __restrict__ a, u32* __restrict__ R, u32 n0,
+                                         u32 count, int vec, u32* lds32, int tid, int nthreads) {
     constexpr int TN = 32 * NCT;
-    const u32* a = blockIdx.y ? a1 : a0;
-    u32* R = blockIdx.y ? R1 : R0;
     const int L = D.L, K = D.K;
     const u32 N = D.N;
-    const u32 n0 = blockIdx.x * (u32)TN;
     const int cnt = (int)min((u32)TN, count - n0);
     const u32* src = a + (size_t)n0 * L;
     const int words = TN * L;
@@ -341,14 +387,14 @@ __global__ void __launch_bounds__(512) k_rns(DevPlan D, const u32* __restrict__ 
     if (vec && cnt == TN) {
         const uint4* s4 = reinterpret_cast<const uint4*>(src);
         uint4* d4 = reinterpret_cast<uint4*>(lds32);
-        for (int i = (int)threadIdx.x; i < words / 4; i += (int)blockDim.x) d4[i] = s4[i];
+        for (int i = tid; i < words / 4; i += nthreads) d4[i] = s4[i];
     } else {
-        for (int i = (int)threadIdx.x; i < words; i += (int)blockDim.x) lds32[i] = i < cnt * L ? src[i] : 0u;
+        for (int i = tid; i < words; i += nthreads) lds32[i] = i < cnt * L ? src[i] : 0u;
     }
-    if (threadIdx.x < 8) lds32[words + threadIdx.x] = 0x80808080u;
+    if (tid < 8) lds32[words + tid] = 0x80808080u;
     __syncthreads();
 
-    const int lane = (int)threadIdx.x & 63, wave = (int)threadIdx.x >> 6, nw = (int)blockDim.x >> 6;
+    const int lane = tid & 63, wave = tid >> 6, nw = nthreads >> 6;
     const int half = lane >> 5, col = lane & 31;
     auto bfrag = [&](int ct, int ks) {
         const u32* q = lds32 + (32 * ct + col) * L + 8 * ks + 4 * half;
@@ -376,6 +422,14 @@ __global__ void __launch_bounds__(512) k_rns(DevPlan D, const u32* __restrict__ 
     }
 }
 
+template <int NCT>
+__global__ void __launch_bounds__(512) k_rns(DevPlan D, const u32* __restrict__ a0, const u32* __restrict__ a1,
+                                              u32* __restrict__ R0, u32* __restrict__ R1, u32 count, int vec) {
+    extern __shared__ __align__(16) u32 lds32[];
+    rns_tile<NCT>(D, blockIdx.y ? a1 : a0, blockIdx.y ? R1 : R0, blockIdx.x * (u32)(32 * NCT), count, vec, lds32,
+                  (int)threadIdx.x, (int)blockDim.x);
+}
+
 // ---------------------------------------------------------------------------
 // CRT. Block = 4 waves and TN = 32 NCT coefficients.
 //   A: y_j = r_j G_j, stored as 4K bytes per coefficient, re-centred (^0x80).
@@ -388,23 +442,22 @@ __global__ void __launch_bounds__(512) k_rns(DevPlan D, const u32* __restrict__ 
 //   D: coalesced store of the block's contiguous [TN][L] output.
 // ---------------------------------------------------------------------------
 template <int NCT>
-__global__ void __launch_bounds__(256) k_crt(DevPlan D, const u32* __restrict__ R, u32* __restrict__ out,
-                                              const uint2* __restrict__ G, u32 count, int ys) {
-    extern __shared__ __align__(16) unsigned char lds8[];
+__device__ __forceinline__ void crt_tile(const DevPlan& D, const u32* __restrict__ R, u32* __restrict__ out,
+                                         const uint2* __restrict__ G, u32 n0, u32 count, int ys, unsigned char* lds8,
+                                         int tid, int nthreads) {
     constexpr int TN = 32 * NCT;
     const int L = D.L, K = D.K, nslot = L + 3, ZL = L + 2;
     const u32 N = D.N;
     unsigned char* Y = lds8;                                           // [TN][ys]
     long long* Pp = reinterpret_cast<long long*>(lds8 + (size_t)TN * ys);  // [TN][nslot]
-    const u32 n0 = blockIdx.x * (u32)TN;
     const int cnt = (int)min((u32)TN, count - n0);
 
     // y bytes: thread (c, r) covers primes j = r, r + JS, ... for coefficient c,
     // eight loads in flight per batch (lanes sweep c, so loads coalesce).
     {
         const int kp = 8 * D.crt_ks;
-        constexpr int JS = 256 / TN;
-        const int c = (int)threadIdx.x % TN, r = (int)threadIdx.x / TN;
+        const int JS = nthreads / TN;
+        const int c = tid % TN, r = tid / TN;
         const bool live = c < cnt;
         const u32* Rc = R + n0 + c;
         for (int j0 = r; j0 < kp; j0 += 8 * JS) {
@@ -430,7 +483,7 @@ __global__ void __launch_bounds__(256) k_crt(DevPlan D, const u32* __restrict__ 
     }
     __syncthreads();
 
-    const int lane = (int)threadIdx.x & 63, wave = (int)threadIdx.x >> 6, nw = (int)blockDim.x >> 6;
+    const int lane = tid & 63, wave = tid >> 6, nw = nthreads >> 6;
     const int half = lane >> 5, col = lane & 31;
     auto bfrag = [&](int ct, int ks) {
         return *reinterpret_cast<const v4i*>(Y + (32 * ct + col) * ys + 32 * ks + 16 * half);
@@ -453,7 +506,7 @@ __global__ void __launch_bounds__(256) k_crt(DevPlan D, const u32* __restrict__ 
     u32* Z = reinterpret_cast<u32*>(Y);  // [TN][ZL], reuses the y bytes
     const int W = D.W, wl = W >> 5, wb = W & 31;
     const u32* C0 = D.C0;
-    for (int c = (int)threadIdx.x; c < cnt; c += (int)blockDim.x) {
+    for (int c = tid; c < cnt; c += nthreads) {
         const long long* P = Pp + (size_t)c * nslot;
         u32* z = Z + (size_t)c * ZL;
         const u64 F = (u64)P[L + 1] + ((u64)P[L + 2] << 32) + (u64)D.F0;
@@ -514,9 +567,106 @@ __global__ void __launch_bounds__(256) k_crt(DevPlan D, const u32* __restrict__ 
     }
     __syncthreads();
     u32* dst = out + (size_t)n0 * L;
-    for (int idx = (int)threadIdx.x; idx < cnt * L; idx += (int)blockDim.x) {
+    for (int idx = tid; idx < cnt * L; idx += nthreads) {
         const int c = L == 1 ? idx : (int)__umulhi((u32)idx, D.magic_L);  // idx / L
         dst[idx] = Z[(size_t)c * ZL + (idx - c * L)];
+    }
+}
+
+template <int NCT>
+__global__ void __launch_bounds__(256) k_crt(DevPlan D, const u32* __restrict__ R, u32* __restrict__ out,
+                                              const uint2* __restrict__ G, u32 count, int ys) {
+    extern __shared__ __align__(16) unsigned char lds8[];
+    crt_tile<NCT>(D, R, out, G, blockIdx.x * (u32)(32 * NCT), count, ys, lds8, (int)threadIdx.x, (int)blockDim.x);
+}
+
+// ---------------------------------------------------------------------------
+// Batched products: one launch per phase, each over that phase's tiles of
+// every product. Inside a phase tiles are independent, so there is no
+// cross-block wait and no device-scope fence; the launch boundary orders the
+// phases. Each phase kernel has its own register budget (one kernel holding
+// every tile type would run at the largest one's). Two ways to hand out
+// tiles: the hardware dispatcher (one block per tile, ctr == nullptr) or a
+// persistent grid whose blocks claim tiles from an atomic counter.
+// ---------------------------------------------------------------------------
+template <bool INV>
+__device__ __forceinline__ void col_tile_nb(int nb, const DevPlan& D, u32* R, int j, int col0, int tc, u32* lds,
+                                            int tid) {
+    switch (nb) {
+        case 1: col_tile<1, INV>(D, R, j, col0, tc, lds, tid); break;
+        case 2: col_tile<2, INV>(D, R, j, col0, tc, lds, tid); break;
+        case 3: col_tile<3, INV>(D, R, j, col0, tc, lds, tid); break;
+        case 4: col_tile<4, INV>(D, R, j, col0, tc, lds, tid); break;
+        case 5: col_tile<5, INV>(D, R, j, col0, tc, lds, tid); break;
+        case 6: col_tile<6, INV>(D, R, j, col0, tc, lds, tid); break;
+        case 7: col_tile<7, INV>(D, R, j, col0, tc, lds, tid); break;
+        case 8: col_tile<8, INV>(D, R, j, col0, tc, lds, tid); break;
+        default: break;
+    }
+}
+
+__device__ __forceinline__ void row_tile_nb(int nb, const DevPlan& D, u32* RB, const u32* RA, int j, u32 row0, int rpb,
+                                            u32* lds, int tid) {
+    switch (nb) {
+        case 0: row_tile<0, ROW_FUSED>(D, RB, RA, j, row0, rpb, lds, tid); break;
+        case 1: row_tile<1, ROW_FUSED>(D, RB, RA, j, row0, rpb, lds, tid); break;
+        case 2: row_tile<2, ROW_FUSED>(D, RB, RA, j, row0, rpb, lds, tid); break;
+        case 3: row_tile<3, ROW_FUSED>(D, RB, RA, j, row0, rpb, lds, tid); break;
+        case 4: row_tile<4, ROW_FUSED>(D, RB, RA, j, row0, rpb, lds, tid); break;
+        case 5: row_tile<5, ROW_FUSED>(D, RB, RA, j, row0, rpb, lds, tid); break;
+        case 6: row_tile<6, ROW_FUSED>(D, RB, RA, j, row0, rpb, lds, tid); break;
+        case 7: row_tile<7, ROW_FUSED>(D, RB, RA, j, row0, rpb, lds, tid); break;
+        case 8: row_tile<8, ROW_FUSED>(D, RB, RA, j, row0, rpb, lds, tid); break;
+        default: break;
+    }
+}
+
+template <int PH>
+__device__ __forceinline__ void batch_task(const BatchProduct& P, u32 x, u32 y, u32* lds, int tid) {
+    const DevPlan& D = P.D;
+    if constexpr (PH == PH_RNS) {
+        rns_tile<kBatchRnsTile / 32>(D, y ? P.b : P.a, y ? P.R1 : P.R0, x, D.N, 1, lds, tid, kBatchThreads);
+    } else if constexpr (PH == PH_COLF) {
+        col_tile_nb<false>(D.w1, D, (y >> 16) ? P.R1 : P.R0, (int)(y & 0xFFFF), (int)x, P.col_tc, lds, tid);
+    } else if constexpr (PH == PH_ROW) {
+        row_tile_nb(D.w2, D, P.R1, P.R0, (int)y, x, P.row_rpb, lds, tid);
+    } else if constexpr (PH == PH_COLI) {
+        col_tile_nb<true>(D.w1, D, P.R1, (int)y, (int)x, P.col_tc, lds, tid);
+    } else {
+        crt_tile<kBatchCrtTile / 32>(D, P.R1, P.c, D.G, x, D.N, P.crt_ys, reinterpret_cast<unsigned char*>(lds), tid,
+                                     kBatchThreads);
+    }
+}
+
+__global__ void k_batch_reset(u32* ctl, u32 n) {
+    for (u32 i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) ctl[i] = 0;
+}
+
+template <int PH>
+__global__ void __launch_bounds__(kBatchThreads) k_batch_phase(const BatchProduct* __restrict__ prods,
+                                                               const BatchTask* __restrict__ tasks, u32 ntasks,
+                                                               u32* ctr) {
+    extern __shared__ __align__(16) u32 lds[];
+    const int tid = (int)threadIdx.x;
+    if (!ctr) {
+        const BatchTask tk = tasks[blockIdx.x];
+        const u32 pi = __builtin_amdgcn_readfirstlane(tk.prod);
+        const u32 x = __builtin_amdgcn_readfirstlane(tk.x), y = __builtin_amdgcn_readfirstlane(tk.y);
+        batch_task<PH>(prods[pi], x, y, lds, tid);
+        return;
+    }
+    __shared__ u32 sh_t;
+    for (;;) {
+        if (tid == 0) sh_t = atomicAdd(ctr, 1u);
+        __syncthreads();
+        const u32 t = __builtin_amdgcn_readfirstlane(sh_t);
+        if (t >= ntasks) break;
+        const BatchTask tk = tasks[t];
+        const u32 pi = __builtin_amdgcn_readfirstlane(tk.prod);
+        const u32 x = __builtin_amdgcn_readfirstlane(tk.x), y = __builtin_amdgcn_readfirstlane(tk.y);
+        batch_task<PH>(prods[pi], x, y, lds, tid);
+        // the next claim overwrites sh_t and the next tile reuses the LDS
+        __syncthreads();
     }
 }
 
@@ -627,6 +777,25 @@ hipError_t launch_crt(const DevPlan& D, const LaunchCfg& C, const u32* R, u32* o
         case 128: hipLaunchKernelGGL((k_crt<4>), grid, block, C.crt_lds, s, D, R, out, G, count, C.crt_ys); break;
         case 64: hipLaunchKernelGGL((k_crt<2>), grid, block, C.crt_lds, s, D, R, out, G, count, C.crt_ys); break;
         case 32: hipLaunchKernelGGL((k_crt<1>), grid, block, C.crt_lds, s, D, R, out, G, count, C.crt_ys); break;
+        default: return hipErrorInvalidValue;
+    }
+    return hipGetLastError();
+}
+
+hipError_t launch_batch_reset(u32* ctl, u32 nctl, hipStream_t s) {
+    hipLaunchKernelGGL(k_batch_reset, dim3((nctl + 255) / 256), dim3(256), 0, s, ctl, nctl);
+    return hipGetLastError();
+}
+
+hipError_t launch_batch_phase(int phase, const BatchProduct* prods, const BatchTask* tasks, u32 ntasks, u32* ctr,
+                              int grid, size_t lds, hipStream_t s) {
+    if (ntasks == 0) return hipSuccess;
+    const dim3 g(ctr ? (u32)grid : ntasks), b(kBatchThreads);
+    switch (phase) {
+#define PM_PH(n) \
+    case n: hipLaunchKernelGGL((k_batch_phase<n>), g, b, lds, s, prods, tasks, ntasks, ctr); break;
+        PM_PH(PH_RNS) PM_PH(PH_COLF) PM_PH(PH_ROW) PM_PH(PH_COLI) PM_PH(PH_CRT)
+#undef PM_PH
         default: return hipErrorInvalidValue;
     }
     return hipGetLastError();

@@ -49,7 +49,24 @@ struct HostPlan {
 // the reference's messages (solve.cu:873-913).
 void plan_point(HostPlan& P, uint32_t N, uint32_t W, uint32_t L, const uint32_t* q, size_t q_limbs);
 
-// Builds the NTT, RNS and CRT tables of a planned point.
-void build_tables(HostPlan& P);
+// Builds the RNS and CRT tables of a planned point, and its own NTT twiddles
+// unless `twiddles` is false (batched products share one table).
+void build_tables(HostPlan& P, bool twiddles = true);
+
+// The prime set shared by every product of a batch: p = 1 + k 2^17 < 2^31 for
+// descending k, so p = 1 (mod 2N) for every N <= 65536, with psi of order
+// 2^17. One twiddle table then serves every N: psi_N = psi^(65536/N) gives
+// psi_N^bitrev_logN(k) = psi^bitrev_16(k) for k < N.
+struct SharedPrimes {
+    std::vector<uint32_t> p, pinv, psi;
+};
+constexpr uint32_t kSharedN = 65536;
+void shared_primes(SharedPrimes& S, int count);
+// Plans a product over the shared primes: the same validation and K rule as
+// plan_point (the first K primes with sum log2 p >= 2W + logN + 3).
+void plan_point_shared(HostPlan& P, uint32_t N, uint32_t W, uint32_t L, const uint32_t* q, size_t q_limbs,
+                       const SharedPrimes& S);
+// tw[j][k] (Shoup pairs) for j < K, k < 65536 over the shared primes.
+void build_shared_twiddles(const SharedPrimes& S, int K, std::vector<uint32_t>& tw);
 
 }  // namespace pm

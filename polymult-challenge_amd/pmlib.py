@@ -23,6 +23,8 @@ C_SYMBOLS = (
     "pm_init", "pm_run", "pm_run_device", "pm_free", "pm_last_error", "pm_query", "pm_point_info",
     "pm_dev_alloc", "pm_dev_free", "pm_copy_to_device", "pm_copy_to_host", "pm_sync", "pm_time_device",
     "pm_stage_rns", "pm_stage_ntt_fwd", "pm_stage_ntt_inv", "pm_stage_crt", "pm_probe_transfers",
+    "pm_batch_create", "pm_batch_free", "pm_batch_last_error", "pm_batch_product_info", "pm_batch_run",
+    "pm_batch_sync", "pm_batch_time", "pm_batch_autotune",
 )
 # The C++ entry points of include/fherma.h, mangled as the reference's harness binds them.
 CXX_SYMBOLS = (
@@ -45,6 +47,7 @@ class PmInfo(ctypes.Structure):
 _LIB = None
 _u32p = ctypes.POINTER(ctypes.c_uint32)
 _vp = ctypes.c_void_p
+_vpp = ctypes.POINTER(ctypes.c_void_p)
 
 
 def load() -> ctypes.CDLL:
@@ -76,6 +79,14 @@ def load() -> ctypes.CDLL:
         "pm_stage_crt": ([_vp, _vp, _vp], ctypes.c_int),
         "pm_probe_transfers": ([ctypes.c_size_t, ctypes.c_int, ctypes.c_int, ctypes.POINTER(ctypes.c_double)],
                                ctypes.c_int),
+        "pm_batch_create": ([ctypes.c_uint32, _u32p, _u32p, _u32p, ctypes.POINTER(_vp)], ctypes.c_int),
+        "pm_batch_free": ([_vp], None),
+        "pm_batch_last_error": ([], ctypes.c_char_p),
+        "pm_batch_product_info": ([_vp, ctypes.c_uint32, ctypes.POINTER(PmInfo)], ctypes.c_int),
+        "pm_batch_run": ([_vp, _vpp, _vpp, _vpp, _vp], ctypes.c_int),
+        "pm_batch_sync": ([_vp], ctypes.c_int),
+        "pm_batch_time": ([_vp, _vpp, _vpp, _vpp, ctypes.c_int, ctypes.POINTER(ctypes.c_float)], ctypes.c_int),
+        "pm_batch_autotune": ([_vp, _vpp, _vpp, _vpp, ctypes.c_int, ctypes.POINTER(ctypes.c_int)], ctypes.c_int),
     }
     for name, (args, res) in sig.items():
         f = getattr(lib, name)
@@ -229,3 +240,66 @@ class Multiplier:
         dc = DeviceBuffer(self.N * self.L * 4)
         _check(load().pm_stage_crt(self._h, dX.ptr, dc.ptr))
         return dc.download((self.N, self.L))
+
+
+def _check_batch(rc: int):
+    if rc != 0:
+        raise PolymultError(load().pm_batch_last_error().decode())
+
+
+class Batch:
+    """Many products of uneven N/W in one persistent launch (pm_batch_*).
+
+    products: list of (N, W, q) with q a Python int or a uint32 limb array."""
+
+    def __init__(self, products):
+        lib = load()
+        Ns = np.array([p[0] for p in products], dtype=np.uint32)
+        Ws = np.array([p[1] for p in products], dtype=np.uint32)
+        qs = []
+        for N, W, q in products:
+            L = (W + 31) // 32
+            if isinstance(q, (int, np.integer)):
+                q = np.frombuffer(int(q).to_bytes(4 * L, "little"), dtype="<u4")
+            qs.append(np.ascontiguousarray(q, dtype=np.uint32).reshape(L))
+        qc = np.ascontiguousarray(np.concatenate(qs))
+        h = _vp()
+        _check_batch(lib.pm_batch_create(len(products), _ptr(Ns), _ptr(Ws), _ptr(qc), ctypes.byref(h)))
+        self._h = h
+        self.count = len(products)
+        self.info = []
+        for i in range(self.count):
+            inf = PmInfo()
+            _check_batch(lib.pm_batch_product_info(h, i, ctypes.byref(inf)))
+            self.info.append(inf)
+
+    def close(self):
+        if getattr(self, "_h", None):
+            load().pm_batch_free(self._h)
+            self._h = None
+
+    def __del__(self):
+        self.close()
+
+    @staticmethod
+    def _ptrs(bufs):
+        arr = (ctypes.c_void_p * len(bufs))(*[b.ptr.value if isinstance(b.ptr, ctypes.c_void_p) else b.ptr
+                                              for b in bufs])
+        return arr
+
+    def run(self, da, db, dc, stream=None):
+        _check_batch(load().pm_batch_run(self._h, self._ptrs(da), self._ptrs(db), self._ptrs(dc), stream))
+
+    def sync(self):
+        _check_batch(load().pm_batch_sync(self._h))
+
+    def time(self, da, db, dc, iters: int) -> float:
+        ms = ctypes.c_float()
+        _check_batch(load().pm_batch_time(self._h, self._ptrs(da), self._ptrs(db), self._ptrs(dc), iters,
+                                          ctypes.byref(ms)))
+        return ms.value
+
+    def autotune(self, da, db, dc, iters: int = 3) -> dict:
+        best = (ctypes.c_int * 4)()
+        _check_batch(load().pm_batch_autotune(self._h, self._ptrs(da), self._ptrs(db), self._ptrs(dc), iters, best))
+        return {"col_tile": best[0], "ceil_split": best[1], "blocks_per_cu": best[2], "us_per_launch": best[3]}

@@ -1,0 +1,86 @@
+"""Batched form: many products of uneven N/W in one persistent launch.
+
+Every product of a batch must come out bit-exact against the oracle (which
+picks its own primes; the result c is canonical, so the shared prime set of
+the batch must not change a single bit), and equal to the plain per-call path.
+"""
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+
+def _batch(O, pm, points, seed, kind="uniform"):
+    prods, ops = [], []
+    for i, (N, W) in enumerate(points):
+        q = O.spec_q(N, W)
+        prods.append((N, W, q))
+        ops.append(O.operands(kind, N, W, q, seed + i))
+    B = pm.Batch(prods)
+    da = [pm.DeviceBuffer.from_array(a) for a, _ in ops]
+    db = [pm.DeviceBuffer.from_array(b) for _, b in ops]
+    dc = [pm.DeviceBuffer(N * O.limbs_of(W) * 4) for N, W in points]
+    return B, prods, ops, da, db, dc
+
+
+def _check(O, prods, ops, dc, idx=None):
+    for i in (range(len(prods)) if idx is None else idx):
+        N, W, q = prods[i]
+        got = dc[i].download((N, O.limbs_of(W)))
+        want = O.OraclePlan(N, W, q).multiply(*ops[i])
+        assert np.array_equal(got, want), "product %d (N=%d W=%d)" % (i, N, W)
+
+
+def test_batch_uneven_small(O, pm):
+    points = [(64, 32), (1024, 512), (4096, 256), (2048, 868), (256, 1747), (512, 4000), (1, 8), (16, 33),
+              (8192, 438), (2, 40)]
+    print("batch of %d uneven products" % len(points), flush=True)
+    B, prods, ops, da, db, dc = _batch(O, pm, points, 100, "mixed")
+    B.run(da, db, dc)
+    B.sync()
+    _check(O, prods, ops, dc)
+    # same result on a second launch (graph replay) and against the plain path
+    for d in dc:
+        d.upload(np.zeros(d.nbytes // 4, np.uint32))
+    B.run(da, db, dc)
+    B.sync()
+    _check(O, prods, ops, dc)
+    for i in (1, 3, 8):
+        N, W, q = prods[i]
+        with pm.Multiplier(N, W, O.q_limbs(q, O.limbs_of(W))) as m:
+            assert np.array_equal(m.run(*ops[i]), dc[i].download((N, O.limbs_of(W))))
+
+
+def test_batch_log_uniform_mix(O, pm):
+    """The measured batch shape: N log-uniform in 2^11..2^16, W uniform in [128, 2048]."""
+    rng = np.random.default_rng(4)
+    points = [(int(2 ** rng.integers(11, 17)), int(rng.integers(128, 2049))) for _ in range(20)]
+    print("batch", points, flush=True)
+    B, prods, ops, da, db, dc = _batch(O, pm, points, 200)
+    B.run(da, db, dc)
+    B.sync()
+    _check(O, prods, ops, dc)
+    best = B.autotune(da, db, dc, 2)
+    print("autotune", best, flush=True)
+    assert best["us_per_launch"] > 0
+    for d in dc:
+        d.upload(np.zeros(d.nbytes // 4, np.uint32))
+    B.run(da, db, dc)
+    B.sync()
+    _check(O, prods, ops, dc)
+
+
+def test_batch_closed_form_full_size(O, pm):
+    """Eight N=65536/W=1747 products of all-(q-1) operands: c_k = (2k + 2 - N) mod q each."""
+    N, W = 65536, 1747
+    q = O.spec_q(N, W)
+    L = O.limbs_of(W)
+    am = np.tile(O.q_limbs(q - 1, L), (N, 1))
+    B = pm.Batch([(N, W, q)] * 8)
+    da = [pm.DeviceBuffer.from_array(am) for _ in range(8)]
+    dc = [pm.DeviceBuffer(N * L * 4) for _ in range(8)]
+    B.run(da, da, dc)
+    B.sync()
+    want = O.all_qm1(N, q)
+    for i in range(8):
+        assert O.unpack(dc[i].download((N, L))) == want, i
