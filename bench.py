@@ -8,7 +8,7 @@ through the C ABI (pm_run_device). `value` is multiplies per second over the
 timed steps. The PCIe-inclusive rate of the host-buffer call (pm_run, the
 fherma_run replacement) is reported beside it, never as `value`.
 
-    python bench.py [--gpus 1] [--steps 200] [--warmup 20] [--no-cpu]
+    python bench.py [--gpus 1] [--steps 200] [--warmup 20] [--no-cpu] [--serve-seconds 10] [--no-serve]
 
 Prints one JSON line.
 """
@@ -109,6 +109,26 @@ def batched(pmlib, O, count: int, points, seed: int, check: int, iters: int) -> 
             "io_bytes": io, "bit_exact_checked": min(check, count), "bit_exact": exact, "autotune": tuned}
 
 
+SERVE_POINTS = [(8192, 438), (32768, 868), (65536, 1747)]
+SERVE_MIX = [0.70, 0.25, 0.05]
+
+
+def serving(pmlib, O, rates, seconds: float) -> dict:
+    """Open-loop Poisson load on the serving mix through the native driver
+    (pm_server_drive); latency from each scheduled arrival to its result in
+    the caller's buffer. One server, one run per rate."""
+    cat = [(n, w, O.spec_q(n, w)) for n, w in SERVE_POINTS]
+    out = {"mix": {"%d/%d" % p: f for p, f in zip(SERVE_POINTS, SERVE_MIX)}, "seconds_per_rate": seconds,
+           "bytes_per_request_mean": round(sum(f * 12 * n * O.limbs_of(w) for (n, w), f in zip(SERVE_POINTS, SERVE_MIX))),
+           "rates": []}
+    with pmlib.Server(cat) as S:
+        S.drive(200.0, 0.5, SERVE_MIX, seed=SEED)  # warm-up
+        for r in rates:
+            rep = S.drive(float(r), seconds, SERVE_MIX, seed=SEED + int(r))
+            out["rates"].append({k: (round(v, 3) if isinstance(v, float) else v) for k, v in rep.items()})
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -117,6 +137,8 @@ def main():
     ap.add_argument("--e2e-calls", type=int, default=20)
     ap.add_argument("--cpu-cases", type=int, default=3)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--serve-seconds", type=float, default=10.0)
+    ap.add_argument("--no-serve", action="store_true")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: this build runs on a single MI355X (the path would shard by RNS "
@@ -190,6 +212,8 @@ def main():
     batch_mix = batched(pmlib, O, 64, mix, SEED + 200, 4, 20)
     batch_mix["workload"] = "64 products, N log-uniform in 2^11..2^16, W uniform in [128, 2048], seed %d" % SEED
 
+    serve = None if args.no_serve else serving(pmlib, O, (200, 1000, 5000), args.serve_seconds)
+
     cpu = None if args.no_cpu else cpu_baseline(q, args.cpu_cases)
     out = {
         "metric": METRIC, "value": round(value, 2), "unit": "mults/s", "n_gpus": 1, "steps": args.steps,
@@ -201,7 +225,7 @@ def main():
         "bit_exact": correct,
         "pct_hbm_peak": round(100 * pcie_bytes / (ms_step * 1e-3) / (HBM_PEAK_GBS * 1e9), 3),
         "roofline": roofline, "kernels": per_kernel, "end_to_end": end_to_end,
-        "batched_same_point": batch_same, "batched_mixed": batch_mix, "cpu_baseline": cpu,
+        "batched_same_point": batch_same, "batched_mixed": batch_mix, "serving": serve, "cpu_baseline": cpu,
     }
     print(json.dumps(out))
 

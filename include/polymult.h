@@ -96,7 +96,7 @@ int pm_stage_ntt_fwd(pm_state* s, uint32_t* d_R);
 int pm_stage_ntt_inv(pm_state* s, uint32_t* d_R);
 int pm_stage_crt(pm_state* s, const uint32_t* d_X, uint32_t* d_c);
 
-/* ---- batched form: many products of uneven N and W in one persistent launch ----
+/* ---- batched form: many products of uneven N and W in one graph launch ----
  * No counterpart in the reference (it multiplies one pair per call). Products
  * share one RNS prime set (p = 1 mod 2^17) and one twiddle table; their K
  * follows the same rule as pm_init. q_concat holds each product's q (L_i =
@@ -107,20 +107,71 @@ void pm_batch_free(pm_batch* b);
 const char* pm_batch_last_error(void);
 int pm_batch_product_info(const pm_batch* b, uint32_t i, pm_info* out);
 /* Enqueues every product (device buffers, N_i*L_i u32 each, 16-byte aligned)
- * as one hipGraph launch on `stream` (NULL: the batch's own). */
+ * as one hipGraph launch on `stream` (NULL: the batch's own): five phase
+ * launches, each over all products' tiles. */
 int pm_batch_run(pm_batch* b, const uint32_t* const* d_a, const uint32_t* const* d_b, uint32_t* const* d_c,
                  void* stream);
-/* Waits for the device and reports a failed batch (a task wait that hit its
- * bound). */
+/* Waits for the device (and so for every launch of the batch). */
 int pm_batch_sync(pm_batch* b);
 /* Average milliseconds of one batch launch over `iters` graph replays. */
 int pm_batch_time(pm_batch* b, const uint32_t* const* d_a, const uint32_t* const* d_b, uint32_t* const* d_c,
                   int iters, float* ms);
 /* Times every candidate (column tile 8/16/32, 4-step split of odd logN,
- * 1 or 2 resident blocks per CU), keeps the fastest; best[0..3] = column
- * tile, split, blocks per CU, microseconds per launch. */
+ * tiles handed out by the dispatcher (0) or by 2 or 4 persistent blocks per
+ * CU), keeps the fastest; best[0..3] = column tile, split, blocks per CU,
+ * microseconds per launch. */
 int pm_batch_autotune(pm_batch* b, const uint32_t* const* d_a, const uint32_t* const* d_b, uint32_t* const* d_c,
                       int iters, int* best);
+
+/* ---- serving: requests from host buffers, continuous batching ----
+ * No counterpart in the reference (its harness calls fherma_run once per
+ * case). A server holds a catalog of points (shared prime set, as the batched
+ * form). Requests name a point and pass pageable host buffers (N*L u32 each;
+ * the caller keeps them alive until pm_server_wait reports the request done).
+ * A scheduler thread groups the waiting requests into one batch whenever a
+ * batch slot is free, admitting them in arrival order while their device
+ * footprint (2 N L + 2 K N words) and pinned staging (2 N L words) fit the
+ * budgets; a request larger than a budget is refused at submission. */
+typedef struct pm_server pm_server;
+typedef struct {
+    uint64_t device_budget; /* HBM bytes for requests in flight (0: 4 GiB) */
+    uint64_t pinned_budget; /* pinned staging bytes (0: 1 GiB) */
+    uint32_t max_batch;     /* requests per batch (0: 64) */
+    uint32_t max_inflight;  /* batches in flight, at most 8 (0: 2) */
+    uint32_t host_threads;  /* staging threads incl. the scheduler (0: min(8, cpus - 2)) */
+    int32_t col_tile;       /* column tile 8, 16 or 32 (0: 32) */
+} pm_server_config;
+typedef struct {
+    uint64_t submitted, completed, failed, batches;
+    uint64_t deferred; /* times the queue head waited for memory */
+    double mean_batch, p50_ms, p99_ms, max_ms, mean_ms;
+    uint64_t device_budget, device_peak, pinned_budget, pinned_peak;
+} pm_server_stats;
+typedef struct {
+    double offered_rate, seconds;
+    uint64_t requests, completed, failed;
+    double achieved_rate, p50_ms, p99_ms, max_ms, mean_ms, mean_batch;
+    uint64_t batches, deferred, device_peak;
+    double max_late_ms; /* worst delay of a submission behind its arrival time */
+} pm_serve_report;
+int pm_server_create(uint32_t npoints, const uint32_t* N, const uint32_t* W, const uint32_t* q_concat,
+                     const pm_server_config* cfg, pm_server** out);
+/* Finishes every submitted request, then stops. */
+void pm_server_free(pm_server* s);
+const char* pm_server_last_error(void);
+int pm_server_submit(pm_server* s, uint32_t point, const uint32_t* a, const uint32_t* b, uint32_t* c,
+                     uint64_t* ticket);
+/* 0: done (the ticket is consumed; latency from submission to c written),
+ * 1: not done within timeout_s, -1: error (the request failed, or a bad
+ * ticket). */
+int pm_server_wait(pm_server* s, uint64_t ticket, double timeout_s, double* latency_ms);
+/* Counters and latency percentiles since the last reset. */
+int pm_server_stats_get(pm_server* s, int reset, pm_server_stats* out);
+/* Open-loop Poisson load from the calling thread: arrivals at `rate` per
+ * second for `seconds`, point i drawn with weight weights[i], operands
+ * generated from `seed`; latency counts from each scheduled arrival. */
+int pm_server_drive(pm_server* s, double rate, double seconds, const double* weights, uint64_t seed,
+                    pm_serve_report* out);
 
 #ifdef __cplusplus
 }

@@ -33,6 +33,15 @@ void ensure(T*& p, size_t& cap, size_t n, const char* what) {
     hip_check(hipMalloc(&p, cap * sizeof(T)), what);
 }
 
+template <class T>
+void ensure_pinned(T*& p, size_t& cap, size_t n) {
+    if (n <= cap) return;
+    if (p) hip_check(hipHostFree(p), "hipHostFree");
+    p = nullptr;
+    cap = std::max<size_t>(n, cap * 2);
+    hip_check(hipHostMalloc(reinterpret_cast<void**>(&p), cap * sizeof(T), hipHostMallocDefault), "hipHostMalloc");
+}
+
 }  // namespace
 
 // ---------------------------------------------------------------------------
@@ -194,13 +203,23 @@ void Runner::plan(const BatchConfig& c) {
     off[PH_COUNT] = (uint32_t)tasks.size();
 }
 
-void Runner::upload(hipStream_t s) {
+void Runner::upload(hipStream_t s, bool pinned) {
     ensure(d_prods, prod_cap, prods.size(), "hipMalloc products");
     ensure(d_tasks, task_cap, tasks.size(), "hipMalloc tiles");
     if (!d_ctr) hip_check(hipMalloc(&d_ctr, PH_COUNT * sizeof(uint32_t)), "hipMalloc counters");
-    hip_check(hipMemcpyAsync(d_prods, prods.data(), prods.size() * sizeof(BatchProduct), hipMemcpyHostToDevice, s),
+    const void* hp = prods.data();
+    const void* ht = tasks.data();
+    if (pinned) {
+        ensure_pinned(h_prods, h_prod_cap, prods.size());
+        ensure_pinned(h_tasks, h_task_cap, tasks.size());
+        std::memcpy(h_prods, prods.data(), prods.size() * sizeof(BatchProduct));
+        std::memcpy(h_tasks, tasks.data(), tasks.size() * sizeof(BatchTask));
+        hp = h_prods;
+        ht = h_tasks;
+    }
+    hip_check(hipMemcpyAsync(d_prods, hp, prods.size() * sizeof(BatchProduct), hipMemcpyHostToDevice, s),
               "upload products");
-    hip_check(hipMemcpyAsync(d_tasks, tasks.data(), tasks.size() * sizeof(BatchTask), hipMemcpyHostToDevice, s),
+    hip_check(hipMemcpyAsync(d_tasks, ht, tasks.size() * sizeof(BatchTask), hipMemcpyHostToDevice, s),
               "upload tiles");
 }
 
@@ -223,6 +242,8 @@ Runner::~Runner() {
     if (d_prods) (void)hipFree(d_prods);
     if (d_tasks) (void)hipFree(d_tasks);
     if (d_ctr) (void)hipFree(d_ctr);
+    if (h_prods) (void)hipHostFree(h_prods);
+    if (h_tasks) (void)hipHostFree(h_tasks);
 }
 
 // ---------------------------------------------------------------------------

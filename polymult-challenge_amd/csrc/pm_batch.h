@@ -51,6 +51,10 @@ struct Runner {
     BatchTask* d_tasks = nullptr;
     size_t task_cap = 0;
     uint32_t* d_ctr = nullptr;  // [PH_COUNT] tile counters of the persistent mode
+    BatchProduct* h_prods = nullptr;  // pinned mirrors
+    size_t h_prod_cap = 0;
+    BatchTask* h_tasks = nullptr;
+    size_t h_task_cap = 0;
     BatchConfig cfg;
     int cus = 0, max_lds = 0;
 
@@ -58,8 +62,11 @@ struct Runner {
     // Sets every product's split and tile shapes and builds the tile lists and
     // the LDS sizes. Throws if a product cannot fit the LDS.
     void plan(const BatchConfig& c);
-    // Copies products and tiles to the device (stream-ordered).
-    void upload(hipStream_t s);
+    // Copies products and tiles to the device (stream-ordered). With
+    // `pinned`, through pinned mirrors, so the call does not wait for the
+    // copy; the mirrors are rewritten by the next upload, so the caller must
+    // know the previous one has completed.
+    void upload(hipStream_t s, bool pinned = false);
     // Enqueues the phases (and the counter reset of the persistent mode).
     hipError_t launch(hipStream_t s) const;
     ~Runner();

@@ -38,6 +38,7 @@
 #include <sched.h>
 #endif
 
+#include "pm_host.h"
 #include "pm_state.h"
 
 namespace pm {
@@ -48,15 +49,6 @@ void hip_check(hipError_t e, const char* what) {
     if (e != hipSuccess) throw std::runtime_error(std::string(what) + ": " + hipGetErrorString(e));
 }
 
-inline void relax() {
-#if defined(__x86_64__)
-    _mm_pause();
-#endif
-}
-
-// Copy that streams the destination past the caches (the staging buffers are
-// written once and read by DMA; the result is read later by the caller), so a
-// copy costs one read and one write of memory rather than three.
 #if defined(__x86_64__)
 __attribute__((target("avx2"))) void copy_stream_avx2(unsigned char* d, const unsigned char* s, size_t n) {
     while (n && ((uintptr_t)d & 31)) {
@@ -79,6 +71,11 @@ __attribute__((target("avx2"))) void copy_stream_avx2(unsigned char* d, const un
 }
 #endif
 
+}  // namespace
+
+// The destination is streamed past the caches (the staging buffers are written
+// once and read by DMA; the result is read later by the caller), so a copy
+// costs one read and one write of memory rather than three.
 void copy_stream(void* dst, const void* src, size_t n) {
 #if defined(__x86_64__)
     static const bool avx2 = __builtin_cpu_supports("avx2");
@@ -97,93 +94,6 @@ int host_cpus() {
 #endif
     return (int)std::max(1u, std::thread::hardware_concurrency());
 }
-
-}  // namespace
-
-// A team of host threads that runs fn(0..n-1) with the calling thread taking
-// part. Between calls the workers sleep; while a multiply is in progress
-// (hot()) they spin so a copy job starts within microseconds.
-class Team {
-  public:
-    explicit Team(int workers) {
-        for (int i = 0; i < workers; ++i) threads_.emplace_back([this] { loop(); });
-    }
-    ~Team() {
-        {
-            std::lock_guard<std::mutex> lk(m_);
-            stop_ = true;
-        }
-        cv_.notify_all();
-        for (auto& t : threads_) t.join();
-    }
-    int size() const { return (int)threads_.size() + 1; }
-    void hot(bool on) {
-        {
-            std::lock_guard<std::mutex> lk(m_);
-            hot_ = on;
-        }
-        if (on) cv_.notify_all();
-    }
-    void run(int n, const std::function<void(int)>& fn) {
-        Job job{&fn, n};
-        current_.store(&job);
-        epoch_.fetch_add(1);
-        if (sleeping_.load() > 0) {
-            // taking the lock orders the wake-up after any sleeper's predicate check
-            { std::lock_guard<std::mutex> lk(m_); }
-            cv_.notify_all();
-        }
-        work(job);
-        while (job.done.load() < n) relax();
-        current_.store(nullptr);
-        while (busy_.load() > 0) relax();  // nobody still holds &job
-    }
-
-  private:
-    struct Job {
-        const std::function<void(int)>* fn;
-        int total;
-        std::atomic<int> next{0};
-        std::atomic<int> done{0};
-    };
-    static void work(Job& j) {
-        for (int i; (i = j.next.fetch_add(1)) < j.total;) {
-            (*j.fn)(i);
-            j.done.fetch_add(1);
-        }
-    }
-    void loop() {
-        uint64_t seen = epoch_.load();
-        for (;;) {
-            int idle = 0;
-            while (epoch_.load() == seen) {
-                relax();
-                if (++idle > 4096) {
-                    std::unique_lock<std::mutex> lk(m_);
-                    if (stop_) return;
-                    if (!hot_ && epoch_.load() == seen) {
-                        sleeping_.fetch_add(1);
-                        cv_.wait(lk, [&] { return stop_ || hot_ || epoch_.load() != seen; });
-                        sleeping_.fetch_sub(1);
-                        if (stop_) return;
-                    }
-                    idle = 0;
-                }
-            }
-            seen = epoch_.load();
-            busy_.fetch_add(1);
-            if (Job* j = current_.load()) work(*j);
-            busy_.fetch_sub(1);
-        }
-    }
-    std::vector<std::thread> threads_;
-    std::atomic<Job*> current_{nullptr};
-    std::atomic<uint64_t> epoch_{0};
-    std::atomic<int> busy_{0}, sleeping_{0};
-    std::mutex m_;
-    std::condition_variable cv_;
-    bool hot_ = false, stop_ = false;
-};
 
 struct Pipeline {
     Team* team = nullptr;

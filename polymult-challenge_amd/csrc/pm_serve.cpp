@@ -1,0 +1,690 @@
+// Serving loop: multiply requests for a fixed catalog of points arrive one at a
+// time from host buffers, and a scheduler thread groups whatever is waiting
+// into a batch (continuous batching). A request is admitted into a batch only
+// once its HBM footprint (operands, result, residues) and its pinned staging
+// fit the server's budgets; requests are admitted in arrival order. Each batch
+// runs as
+//   stage (host team: pageable -> pinned) -> one H2D per request ->
+//   five phase launches over every request's tiles -> one D2H per request ->
+//   scatter (host team: pinned -> the caller's buffer),
+// on three streams (in, compute, out) with up to `max_inflight` batches at
+// once, so one batch's transfers overlap another's compute. Latency is
+// measured from a request's arrival to the end of its scatter.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <condition_variable>
+#include <deque>
+#include <map>
+#include <memory>
+#include <mutex>
+#include <random>
+#include <string>
+#include <thread>
+#include <unordered_map>
+#include <vector>
+
+#include "../../include/polymult.h"
+#include "pm_batch.h"
+#include "pm_host.h"
+
+namespace pm {
+namespace {
+
+using Clock = std::chrono::steady_clock;
+double now_s() { return std::chrono::duration<double>(Clock::now().time_since_epoch()).count(); }
+size_t round256(size_t n) { return (n + 255) & ~size_t(255); }
+
+// Nearest-rank percentile of a sorted, non-empty sample.
+double percentile(const std::vector<double>& v, double p) {
+    size_t i = (size_t)std::ceil(p * (double)v.size());
+    i = i ? i - 1 : 0;
+    return v[std::min(v.size() - 1, i)];
+}
+
+// First-fit allocator over [0, size) in 256-byte granules; free ranges are
+// kept sorted and merged on release.
+class Arena {
+  public:
+    void reset(size_t size) {
+        free_.clear();
+        if (size) free_[0] = size;
+        size_ = size;
+        used_ = peak_ = 0;
+    }
+    bool alloc(size_t n, size_t& off) {
+        n = round256(n);
+        for (auto it = free_.begin(); it != free_.end(); ++it) {
+            if (it->second < n) continue;
+            off = it->first;
+            const size_t rest = it->second - n;
+            free_.erase(it);
+            if (rest) free_[off + n] = rest;
+            used_ += n;
+            peak_ = std::max(peak_, used_);
+            return true;
+        }
+        return false;
+    }
+    void release(size_t off, size_t n) {
+        n = round256(n);
+        used_ -= n;
+        auto it = free_.emplace(off, n).first;
+        auto nx = std::next(it);
+        if (nx != free_.end() && it->first + it->second == nx->first) {
+            it->second += nx->second;
+            free_.erase(nx);
+        }
+        if (it != free_.begin()) {
+            auto pv = std::prev(it);
+            if (pv->first + pv->second == it->first) {
+                pv->second += it->second;
+                free_.erase(it);
+            }
+        }
+    }
+    size_t size() const { return size_; }
+    size_t used() const { return used_; }
+    size_t peak() const { return peak_; }
+    void clear_peak() { peak_ = used_; }
+
+  private:
+    std::map<size_t, size_t> free_;
+    size_t size_ = 0, used_ = 0, peak_ = 0;
+};
+
+struct Request {
+    uint64_t id = 0;
+    uint32_t point = 0;
+    const uint32_t* a = nullptr;
+    const uint32_t* b = nullptr;
+    uint32_t* c = nullptr;
+    double t_arrive = 0, t_done = 0;
+    size_t dev_off = 0, pin_off = 0;
+    int state = 0;  // 0 queued, 1 in a batch, 2 done, -1 failed
+    std::string error;
+};
+
+// A piece of a host copy, so the team can split large operands.
+struct Piece {
+    void* dst;
+    const void* src;
+    size_t n;
+};
+constexpr size_t kPiece = 2u << 20;
+
+void add_pieces(std::vector<Piece>& v, void* dst, const void* src, size_t n) {
+    for (size_t o = 0; o < n; o += kPiece)
+        v.push_back({static_cast<char*>(dst) + o, static_cast<const char*>(src) + o, std::min(kPiece, n - o)});
+}
+
+}  // namespace
+
+struct Server {
+    Catalog cat;
+    BatchConfig cfg;
+    std::vector<size_t> io_bytes, res_bytes, dev_need, pin_need;  // per point
+    void* d_arena = nullptr;
+    unsigned char* h_arena = nullptr;
+    Arena dev_arena, pin_arena;
+    size_t max_batch = 64;
+    hipStream_t s_in = nullptr, s_comp = nullptr, s_out = nullptr;
+    std::unique_ptr<Team> team;
+
+    struct Slot {
+        Runner run;
+        std::vector<Request*> reqs;
+        hipEvent_t ev_in = nullptr, ev_comp = nullptr, ev_out = nullptr;
+    };
+    std::vector<std::unique_ptr<Slot>> slots;
+    std::deque<Slot*> inflight;  // oldest first
+    std::vector<Slot*> idle;
+
+    std::mutex m;
+    std::condition_variable cv_work, cv_done;
+    std::deque<Request*> queue;
+    std::unordered_map<uint64_t, std::unique_ptr<Request>> reqs;
+    uint64_t next_id = 1;
+    bool stop = false;
+    std::string broken;
+    uint64_t last_deferred = 0;
+
+    // statistics since the last reset
+    std::vector<double> lat_ms;
+    uint64_t n_submitted = 0, n_completed = 0, n_failed = 0, n_batches = 0, n_batched = 0, n_deferred = 0;
+
+    std::thread sched;
+
+    void create(uint32_t npoints, const uint32_t* N, const uint32_t* W, const uint32_t* q, const pm_server_config* c);
+    ~Server();
+
+    uint64_t submit(uint32_t point, const uint32_t* a, const uint32_t* b, uint32_t* c, double t_arrive);
+    int wait(uint64_t id, double timeout_s, double* latency_ms);
+    void stats(pm_server_stats* out, bool reset);
+
+  private:
+    void loop();
+    std::vector<Request*> take();
+    void launch(Slot& S);
+    void retire(Slot& S, hipError_t status);
+    void fail_batch(Slot& S, const std::string& why);
+};
+
+void Server::create(uint32_t npoints, const uint32_t* N, const uint32_t* W, const uint32_t* q,
+                    const pm_server_config* c) {
+    pm_server_config C{};
+    if (c) C = *c;
+    const size_t dev_budget = C.device_budget ? C.device_budget : (size_t)4 << 30;
+    const size_t pin_budget = C.pinned_budget ? C.pinned_budget : (size_t)1 << 30;
+    max_batch = C.max_batch ? C.max_batch : 64;
+    const int ninflight = C.max_inflight ? (int)C.max_inflight : 2;
+    if (ninflight > 8) throw std::runtime_error("max_inflight above 8");
+    cfg.col_tc = C.col_tile ? C.col_tile : 32;
+    if (cfg.col_tc != 8 && cfg.col_tc != 16 && cfg.col_tc != 32) throw std::runtime_error("col_tile must be 8, 16 or 32");
+
+    cat.build(npoints, N, W, q);
+    for (uint32_t i = 0; i < npoints; ++i) {
+        io_bytes.push_back(round256(cat.io_words(i) * 4));
+        res_bytes.push_back(round256((size_t)cat.plans[i].K * cat.plans[i].N * 4));
+        dev_need.push_back(2 * io_bytes[i] + 2 * res_bytes[i]);  // a|b (c over a), R0, R1
+        pin_need.push_back(2 * io_bytes[i]);                      // a|b in, c back over a
+    }
+    hip_check(hipMalloc(&d_arena, dev_budget), "hipMalloc request arena");
+    hip_check(hipHostMalloc(reinterpret_cast<void**>(&h_arena), pin_budget, hipHostMallocDefault),
+              "hipHostMalloc staging arena");
+    dev_arena.reset(dev_budget);
+    pin_arena.reset(pin_budget);
+    hip_check(hipStreamCreateWithFlags(&s_in, hipStreamNonBlocking), "stream");
+    hip_check(hipStreamCreateWithFlags(&s_comp, hipStreamNonBlocking), "stream");
+    hip_check(hipStreamCreateWithFlags(&s_out, hipStreamNonBlocking), "stream");
+    for (int i = 0; i < ninflight; ++i) {
+        auto S = std::make_unique<Slot>();
+        S->run.init();
+        hip_check(hipEventCreateWithFlags(&S->ev_in, hipEventDisableTiming), "event");
+        hip_check(hipEventCreateWithFlags(&S->ev_comp, hipEventDisableTiming), "event");
+        hip_check(hipEventCreateWithFlags(&S->ev_out, hipEventDisableTiming), "event");
+        idle.push_back(S.get());
+        slots.push_back(std::move(S));
+    }
+    // Every point must plan on this device (LDS) before a request can arrive.
+    {
+        Runner& R = slots[0]->run;
+        R.prods.assign(npoints, BatchProduct{});
+        for (uint32_t i = 0; i < npoints; ++i) R.prods[i].D = cat.dev[i];
+        R.plan(cfg);
+        R.prods.clear();
+        R.tasks.clear();
+    }
+    const int threads = C.host_threads ? (int)C.host_threads : std::max(1, std::min(8, host_cpus() - 2));
+    team = std::make_unique<Team>(threads - 1);
+    sched = std::thread([this] { loop(); });
+}
+
+Server::~Server() {
+    {
+        std::lock_guard<std::mutex> lk(m);
+        stop = true;
+    }
+    cv_work.notify_all();
+    if (sched.joinable()) sched.join();
+    for (hipStream_t s : {s_in, s_comp, s_out})
+        if (s) (void)hipStreamSynchronize(s);
+    for (auto& S : slots)
+        for (hipEvent_t e : {S->ev_in, S->ev_comp, S->ev_out})
+            if (e) (void)hipEventDestroy(e);
+    slots.clear();
+    for (hipStream_t s : {s_in, s_comp, s_out})
+        if (s) (void)hipStreamDestroy(s);
+    if (d_arena) (void)hipFree(d_arena);
+    if (h_arena) (void)hipHostFree(h_arena);
+}
+
+uint64_t Server::submit(uint32_t point, const uint32_t* a, const uint32_t* b, uint32_t* c, double t_arrive) {
+    if (point >= cat.plans.size()) throw std::runtime_error("point index out of range");
+    if (!a || !b || !c) throw std::runtime_error("null operand pointer");
+    auto r = std::make_unique<Request>();
+    r->point = point;
+    r->a = a;
+    r->b = b;
+    r->c = c;
+    r->t_arrive = t_arrive > 0 ? t_arrive : now_s();
+    uint64_t id;
+    {
+        std::lock_guard<std::mutex> lk(m);
+        if (!broken.empty()) throw std::runtime_error("server stopped after an error: " + broken);
+        if (stop) throw std::runtime_error("server is shutting down");
+        if (dev_need[point] > dev_arena.size())
+            throw std::runtime_error("request needs " + std::to_string(dev_need[point]) +
+                                     " bytes of device memory, above the server's budget of " +
+                                     std::to_string(dev_arena.size()));
+        if (pin_need[point] > pin_arena.size())
+            throw std::runtime_error("request needs " + std::to_string(pin_need[point]) +
+                                     " bytes of pinned staging, above the server's budget of " +
+                                     std::to_string(pin_arena.size()));
+        id = r->id = next_id++;
+        queue.push_back(r.get());
+        reqs.emplace(id, std::move(r));
+        ++n_submitted;
+    }
+    cv_work.notify_one();
+    return id;
+}
+
+int Server::wait(uint64_t id, double timeout_s, double* latency_ms) {
+    std::unique_lock<std::mutex> lk(m);
+    auto it = reqs.find(id);
+    if (it == reqs.end()) throw std::runtime_error("unknown ticket");
+    Request* r = it->second.get();
+    auto ready = [&] { return r->state == 2 || r->state < 0; };
+    if (!ready()) {
+        if (timeout_s <= 0) return 1;
+        if (!cv_done.wait_for(lk, std::chrono::duration<double>(timeout_s), ready)) return 1;
+    }
+    if (latency_ms) *latency_ms = 1e3 * (r->t_done - r->t_arrive);
+    if (r->state < 0) {
+        const std::string e = r->error;
+        reqs.erase(it);
+        throw std::runtime_error(e);
+    }
+    reqs.erase(it);
+    return 0;
+}
+
+void Server::stats(pm_server_stats* out, bool reset) {
+    std::lock_guard<std::mutex> lk(m);
+    *out = pm_server_stats{};
+    out->submitted = n_submitted;
+    out->completed = n_completed;
+    out->failed = n_failed;
+    out->batches = n_batches;
+    out->deferred = n_deferred;
+    out->mean_batch = n_batches ? (double)n_batched / n_batches : 0.0;
+    if (!lat_ms.empty()) {
+        std::vector<double> v = lat_ms;
+        std::sort(v.begin(), v.end());
+        out->p50_ms = percentile(v, 0.50);
+        out->p99_ms = percentile(v, 0.99);
+        out->max_ms = v.back();
+        double s = 0;
+        for (double x : v) s += x;
+        out->mean_ms = s / v.size();
+    }
+    out->device_budget = dev_arena.size();
+    out->device_peak = dev_arena.peak();
+    out->pinned_budget = pin_arena.size();
+    out->pinned_peak = pin_arena.peak();
+    if (reset) {
+        lat_ms.clear();
+        n_submitted = n_completed = n_failed = n_batches = n_batched = n_deferred = 0;
+        dev_arena.clear_peak();
+        pin_arena.clear_peak();
+    }
+}
+
+// Admits requests from the head of the queue while they fit both budgets.
+// Strict arrival order: a head that does not fit holds back the ones behind
+// it (no starvation of large requests). Called with no lock held.
+std::vector<Request*> Server::take() {
+    std::vector<Request*> batch;
+    std::lock_guard<std::mutex> lk(m);
+    while (!queue.empty() && batch.size() < max_batch) {
+        Request* r = queue.front();
+        size_t doff = 0, poff = 0;
+        if (!dev_arena.alloc(dev_need[r->point], doff)) {
+            if (r->id != last_deferred) ++n_deferred, last_deferred = r->id;
+            break;
+        }
+        if (!pin_arena.alloc(pin_need[r->point], poff)) {
+            dev_arena.release(doff, dev_need[r->point]);
+            if (r->id != last_deferred) ++n_deferred, last_deferred = r->id;
+            break;
+        }
+        r->dev_off = doff;
+        r->pin_off = poff;
+        r->state = 1;
+        queue.pop_front();
+        batch.push_back(r);
+    }
+    return batch;
+}
+
+void Server::launch(Slot& S) {
+    unsigned char* dev = static_cast<unsigned char*>(d_arena);
+    // stage a and b of every request into pinned memory
+    std::vector<Piece> pieces;
+    for (Request* r : S.reqs) {
+        const size_t n = cat.io_words(r->point) * 4;
+        add_pieces(pieces, h_arena + r->pin_off, r->a, n);
+        add_pieces(pieces, h_arena + r->pin_off + io_bytes[r->point], r->b, n);
+    }
+    team->run((int)pieces.size(), [&](int i) { copy_stream(pieces[i].dst, pieces[i].src, pieces[i].n); });
+    for (Request* r : S.reqs)
+        hip_check(hipMemcpyAsync(dev + r->dev_off, h_arena + r->pin_off,
+                                 io_bytes[r->point] + cat.io_words(r->point) * 4, hipMemcpyHostToDevice, s_in),
+                  "H2D");
+    hip_check(hipEventRecord(S.ev_in, s_in), "record");
+
+    Runner& R = S.run;
+    R.prods.assign(S.reqs.size(), BatchProduct{});
+    for (size_t i = 0; i < S.reqs.size(); ++i) {
+        const Request* r = S.reqs[i];
+        BatchProduct& P = R.prods[i];
+        P.D = cat.dev[r->point];
+        uint32_t* base = reinterpret_cast<uint32_t*>(dev + r->dev_off);
+        P.a = base;
+        P.b = base + io_bytes[r->point] / 4;
+        P.c = base;  // the CRT phase runs after every RNS tile has read a
+        P.R0 = base + 2 * io_bytes[r->point] / 4;
+        P.R1 = P.R0 + res_bytes[r->point] / 4;
+    }
+    R.plan(cfg);
+    hip_check(hipStreamWaitEvent(s_comp, S.ev_in, 0), "wait");
+    R.upload(s_comp, true);
+    hip_check(R.launch(s_comp), "batch launch");
+    hip_check(hipEventRecord(S.ev_comp, s_comp), "record");
+
+    hip_check(hipStreamWaitEvent(s_out, S.ev_comp, 0), "wait");
+    for (Request* r : S.reqs)
+        hip_check(hipMemcpyAsync(h_arena + r->pin_off, dev + r->dev_off, cat.io_words(r->point) * 4,
+                                 hipMemcpyDeviceToHost, s_out),
+                  "D2H");
+    hip_check(hipEventRecord(S.ev_out, s_out), "record");
+}
+
+void Server::retire(Slot& S, hipError_t status) {
+    if (status != hipSuccess) {
+        fail_batch(S, std::string("batch failed: ") + hipGetErrorString(status));
+        return;
+    }
+    std::vector<Piece> pieces;
+    for (Request* r : S.reqs) add_pieces(pieces, r->c, h_arena + r->pin_off, cat.io_words(r->point) * 4);
+    team->run((int)pieces.size(), [&](int i) { copy_stream(pieces[i].dst, pieces[i].src, pieces[i].n); });
+    const double t = now_s();
+    {
+        std::lock_guard<std::mutex> lk(m);
+        for (Request* r : S.reqs) {
+            r->t_done = t;
+            r->state = 2;
+            lat_ms.push_back(1e3 * (t - r->t_arrive));
+            dev_arena.release(r->dev_off, dev_need[r->point]);
+            pin_arena.release(r->pin_off, pin_need[r->point]);
+        }
+        n_completed += S.reqs.size();
+        n_batches += 1;
+        n_batched += S.reqs.size();
+    }
+    S.reqs.clear();
+    cv_done.notify_all();
+}
+
+void Server::fail_batch(Slot& S, const std::string& why) {
+    const double t = now_s();
+    {
+        std::lock_guard<std::mutex> lk(m);
+        for (Request* r : S.reqs) {
+            r->t_done = t;
+            r->state = -1;
+            r->error = why;
+            dev_arena.release(r->dev_off, dev_need[r->point]);
+            pin_arena.release(r->pin_off, pin_need[r->point]);
+        }
+        n_failed += S.reqs.size();
+        if (broken.empty()) broken = why;
+        // nothing queued can run on a device in an unknown state
+        for (Request* r : queue) {
+            r->t_done = t;
+            r->state = -1;
+            r->error = why;
+            ++n_failed;
+        }
+        queue.clear();
+    }
+    S.reqs.clear();
+    cv_done.notify_all();
+}
+
+void Server::loop() {
+    for (;;) {
+        // retire every finished batch, oldest first
+        while (!inflight.empty()) {
+            Slot* S = inflight.front();
+            const hipError_t q = hipEventQuery(S->ev_out);
+            if (q == hipErrorNotReady) break;
+            inflight.pop_front();
+            retire(*S, q);
+            idle.push_back(S);
+        }
+        // start a batch if a slot is free and something admissible waits
+        if (!idle.empty()) {
+            std::vector<Request*> batch = take();
+            if (!batch.empty()) {
+                Slot* S = idle.back();
+                idle.pop_back();
+                S->reqs = std::move(batch);
+                try {
+                    launch(*S);
+                    inflight.push_back(S);
+                } catch (const std::exception& e) {
+                    // drain what was already issued before reusing anything
+                    for (hipStream_t s : {s_in, s_comp, s_out}) (void)hipStreamSynchronize(s);
+                    fail_batch(*S, e.what());
+                    idle.push_back(S);
+                }
+                continue;
+            }
+        }
+        if (inflight.empty()) {
+            std::unique_lock<std::mutex> lk(m);
+            if (queue.empty()) {
+                if (stop) return;
+                cv_work.wait(lk, [&] { return stop || !queue.empty(); });
+            }
+            continue;
+        }
+        // work in flight and nothing to start: poll again shortly
+        for (int i = 0; i < 64; ++i) relax();
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Poisson load driver
+// ---------------------------------------------------------------------------
+namespace {
+
+// Random operand below 2^(W-1) < q: every limb uniform, the top bits cleared.
+std::vector<uint32_t> random_operand(std::mt19937_64& g, uint32_t N, uint32_t W) {
+    const uint32_t L = (W + 31) / 32;
+    std::vector<uint32_t> v((size_t)N * L);
+    for (uint32_t n = 0; n < N; ++n)
+        for (uint32_t i = 0; i < L; ++i) {
+            const int bits = std::max(0, std::min(32, (int)W - 1 - 32 * (int)i));
+            const uint32_t mask = bits == 32 ? 0xFFFFFFFFu : ((1u << bits) - 1u);
+            v[(size_t)n * L + i] = (uint32_t)g() & mask;
+        }
+    return v;
+}
+
+void drive(Server& S, double rate, double seconds, const double* weights, uint64_t seed, pm_serve_report* out) {
+    if (!(rate > 0) || !(seconds > 0)) throw std::runtime_error("rate and seconds must be positive");
+    const size_t np = S.cat.plans.size();
+    std::vector<double> w(weights, weights + np);
+    double wsum = 0;
+    for (double x : w) {
+        if (!(x >= 0)) throw std::runtime_error("mix weights must be non-negative");
+        wsum += x;
+    }
+    if (!(wsum > 0)) throw std::runtime_error("mix weights sum to zero");
+    std::mt19937_64 g(seed);
+    // two operand pairs per point, alternating over the requests of that point
+    std::vector<std::vector<std::vector<uint32_t>>> ops(np);
+    for (size_t i = 0; i < np; ++i)
+        if (w[i] > 0)
+            for (int k = 0; k < 4; ++k) ops[i].push_back(random_operand(g, S.cat.plans[i].N, S.cat.plans[i].W));
+    std::vector<std::vector<std::unique_ptr<std::vector<uint32_t>>>> pool(np);  // free result buffers
+    std::discrete_distribution<int> pick(w.begin(), w.end());
+    std::exponential_distribution<double> gap(rate);
+
+    struct Out {
+        uint64_t id;
+        uint32_t point;
+        std::unique_ptr<std::vector<uint32_t>> buf;
+    };
+    std::deque<Out> outstanding;
+    std::vector<double> lat;
+    uint64_t failed = 0, nreq = 0;
+    double max_late = 0;
+    auto reap = [&](bool block) {
+        for (auto it = outstanding.begin(); it != outstanding.end();) {
+            double ms = 0;
+            int rc;
+            try {
+                rc = S.wait(it->id, block ? 60.0 : 0.0, &ms);
+            } catch (const std::exception&) {
+                rc = -1;
+            }
+            if (rc == 1) {
+                if (block) throw std::runtime_error("request did not complete within 60 s");
+                ++it;
+                continue;
+            }
+            if (rc == 0)
+                lat.push_back(ms);
+            else
+                ++failed;
+            pool[it->point].push_back(std::move(it->buf));
+            it = outstanding.erase(it);
+        }
+    };
+
+    pm_server_stats st0;
+    S.stats(&st0, true);
+    const double t0 = now_s() + 0.02;
+    double t = 0;
+    for (;;) {
+        t += gap(g);
+        if (t >= seconds) break;
+        const double target = t0 + t;
+        for (double d; (d = target - now_s()) > 0;) {
+            if (d > 300e-6) std::this_thread::sleep_for(std::chrono::duration<double>(d - 200e-6));
+            else relax();
+        }
+        max_late = std::max(max_late, now_s() - target);
+        const uint32_t p = (uint32_t)pick(g);
+        if (pool[p].empty())
+            pool[p].push_back(std::make_unique<std::vector<uint32_t>>(S.cat.io_words(p)));
+        Out o{0, p, std::move(pool[p].back())};
+        pool[p].pop_back();
+        const auto& op = ops[p];
+        const size_t k = (size_t)(nreq % 2) * 2;
+        o.id = S.submit(p, op[k].data(), op[k + 1].data(), o.buf->data(), target);
+        outstanding.push_back(std::move(o));
+        ++nreq;
+        if (outstanding.size() > 8) reap(false);
+    }
+    reap(true);
+    const double t_end = now_s();
+    pm_server_stats st;
+    S.stats(&st, true);
+    *out = pm_serve_report{};
+    out->offered_rate = rate;
+    out->seconds = seconds;
+    out->requests = nreq;
+    out->completed = lat.size();
+    out->failed = failed;
+    out->achieved_rate = lat.size() / std::max(1e-9, t_end - t0);
+    if (!lat.empty()) {
+        std::sort(lat.begin(), lat.end());
+        out->p50_ms = percentile(lat, 0.50);
+        out->p99_ms = percentile(lat, 0.99);
+        out->max_ms = lat.back();
+        double s = 0;
+        for (double x : lat) s += x;
+        out->mean_ms = s / lat.size();
+    }
+    out->mean_batch = st.mean_batch;
+    out->batches = st.batches;
+    out->deferred = st.deferred;
+    out->device_peak = st.device_peak;
+    out->max_late_ms = 1e3 * max_late;
+}
+
+}  // namespace
+}  // namespace pm
+
+// ---------------------------------------------------------------------------
+// C ABI
+// ---------------------------------------------------------------------------
+struct pm_server {
+    std::unique_ptr<pm::Server> s;
+};
+
+namespace {
+thread_local std::string g_server_error;
+template <class F>
+int guarded(F&& body) {
+    try {
+        return body();
+    } catch (const std::exception& e) {
+        g_server_error = e.what();
+    } catch (...) {
+        g_server_error = "unknown error";
+    }
+    return -1;
+}
+}  // namespace
+
+extern "C" {
+
+const char* pm_server_last_error(void) { return g_server_error.c_str(); }
+
+int pm_server_create(uint32_t npoints, const uint32_t* N, const uint32_t* W, const uint32_t* q_concat,
+                     const pm_server_config* cfg, pm_server** out) {
+    return guarded([&] {
+        if (!out || !N || !W || !q_concat) throw std::runtime_error("null argument");
+        *out = nullptr;
+        auto h = std::make_unique<pm_server>();
+        h->s = std::make_unique<pm::Server>();
+        h->s->create(npoints, N, W, q_concat, cfg);
+        *out = h.release();
+        return 0;
+    });
+}
+
+void pm_server_free(pm_server* s) { delete s; }
+
+int pm_server_submit(pm_server* s, uint32_t point, const uint32_t* a, const uint32_t* b, uint32_t* c,
+                     uint64_t* ticket) {
+    return guarded([&] {
+        if (!s || !ticket) throw std::runtime_error("null argument");
+        *ticket = s->s->submit(point, a, b, c, 0.0);
+        return 0;
+    });
+}
+
+int pm_server_wait(pm_server* s, uint64_t ticket, double timeout_s, double* latency_ms) {
+    return guarded([&] {
+        if (!s) throw std::runtime_error("null argument");
+        return s->s->wait(ticket, timeout_s, latency_ms);
+    });
+}
+
+int pm_server_stats_get(pm_server* s, int reset, pm_server_stats* out) {
+    return guarded([&] {
+        if (!s || !out) throw std::runtime_error("null argument");
+        s->s->stats(out, reset != 0);
+        return 0;
+    });
+}
+
+int pm_server_drive(pm_server* s, double rate, double seconds, const double* weights, uint64_t seed,
+                    pm_serve_report* out) {
+    return guarded([&] {
+        if (!s || !weights || !out) throw std::runtime_error("null argument");
+        pm::drive(*s->s, rate, seconds, weights, seed, out);
+        return 0;
+    });
+}
+
+}  // extern "C"

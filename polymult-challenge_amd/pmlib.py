@@ -25,6 +25,8 @@ C_SYMBOLS = (
     "pm_stage_rns", "pm_stage_ntt_fwd", "pm_stage_ntt_inv", "pm_stage_crt", "pm_probe_transfers",
     "pm_batch_create", "pm_batch_free", "pm_batch_last_error", "pm_batch_product_info", "pm_batch_run",
     "pm_batch_sync", "pm_batch_time", "pm_batch_autotune",
+    "pm_server_create", "pm_server_free", "pm_server_last_error", "pm_server_submit", "pm_server_wait",
+    "pm_server_stats_get", "pm_server_drive",
 )
 # The C++ entry points of include/fherma.h, mangled as the reference's harness binds them.
 CXX_SYMBOLS = (
@@ -42,6 +44,31 @@ class PmInfo(ctypes.Structure):
     _fields_ = [("N", ctypes.c_uint32), ("W", ctypes.c_uint32), ("L", ctypes.c_uint32), ("K", ctypes.c_uint32),
                 ("delta", ctypes.c_uint64), ("w1", ctypes.c_uint32), ("w2", ctypes.c_uint32),
                 ("device_bytes", ctypes.c_uint64)]
+
+
+class ServerConfig(ctypes.Structure):
+    _fields_ = [("device_budget", ctypes.c_uint64), ("pinned_budget", ctypes.c_uint64),
+                ("max_batch", ctypes.c_uint32), ("max_inflight", ctypes.c_uint32),
+                ("host_threads", ctypes.c_uint32), ("col_tile", ctypes.c_int32)]
+
+
+class ServerStats(ctypes.Structure):
+    _fields_ = [(n, ctypes.c_uint64) for n in ("submitted", "completed", "failed", "batches", "deferred")] + \
+               [(n, ctypes.c_double) for n in ("mean_batch", "p50_ms", "p99_ms", "max_ms", "mean_ms")] + \
+               [(n, ctypes.c_uint64) for n in ("device_budget", "device_peak", "pinned_budget", "pinned_peak")]
+
+
+class ServeReport(ctypes.Structure):
+    _fields_ = [("offered_rate", ctypes.c_double), ("seconds", ctypes.c_double)] + \
+               [(n, ctypes.c_uint64) for n in ("requests", "completed", "failed")] + \
+               [(n, ctypes.c_double) for n in ("achieved_rate", "p50_ms", "p99_ms", "max_ms", "mean_ms",
+                                               "mean_batch")] + \
+               [(n, ctypes.c_uint64) for n in ("batches", "deferred", "device_peak")] + \
+               [("max_late_ms", ctypes.c_double)]
+
+
+def _as_dict(st: ctypes.Structure) -> dict:
+    return {name: getattr(st, name) for name, _ in st._fields_}
 
 
 _LIB = None
@@ -87,6 +114,16 @@ def load() -> ctypes.CDLL:
         "pm_batch_sync": ([_vp], ctypes.c_int),
         "pm_batch_time": ([_vp, _vpp, _vpp, _vpp, ctypes.c_int, ctypes.POINTER(ctypes.c_float)], ctypes.c_int),
         "pm_batch_autotune": ([_vp, _vpp, _vpp, _vpp, ctypes.c_int, ctypes.POINTER(ctypes.c_int)], ctypes.c_int),
+        "pm_server_create": ([ctypes.c_uint32, _u32p, _u32p, _u32p, ctypes.POINTER(ServerConfig),
+                              ctypes.POINTER(_vp)], ctypes.c_int),
+        "pm_server_free": ([_vp], None),
+        "pm_server_last_error": ([], ctypes.c_char_p),
+        "pm_server_submit": ([_vp, ctypes.c_uint32, _u32p, _u32p, _u32p, ctypes.POINTER(ctypes.c_uint64)],
+                             ctypes.c_int),
+        "pm_server_wait": ([_vp, ctypes.c_uint64, ctypes.c_double, ctypes.POINTER(ctypes.c_double)], ctypes.c_int),
+        "pm_server_stats_get": ([_vp, ctypes.c_int, ctypes.POINTER(ServerStats)], ctypes.c_int),
+        "pm_server_drive": ([_vp, ctypes.c_double, ctypes.c_double, ctypes.POINTER(ctypes.c_double), ctypes.c_uint64,
+                             ctypes.POINTER(ServeReport)], ctypes.c_int),
     }
     for name, (args, res) in sig.items():
         f = getattr(lib, name)
@@ -248,7 +285,7 @@ def _check_batch(rc: int):
 
 
 class Batch:
-    """Many products of uneven N/W in one persistent launch (pm_batch_*).
+    """Many products of uneven N/W in one graph launch (pm_batch_*).
 
     products: list of (N, W, q) with q a Python int or a uint32 limb array."""
 
@@ -303,3 +340,88 @@ class Batch:
         best = (ctypes.c_int * 4)()
         _check_batch(load().pm_batch_autotune(self._h, self._ptrs(da), self._ptrs(db), self._ptrs(dc), iters, best))
         return {"col_tile": best[0], "ceil_split": best[1], "blocks_per_cu": best[2], "us_per_launch": best[3]}
+
+
+def _check_server(rc: int):
+    if rc < 0:
+        raise PolymultError(load().pm_server_last_error().decode())
+    return rc
+
+
+def _q_array(q, L: int) -> np.ndarray:
+    if isinstance(q, (int, np.integer)):
+        q = np.frombuffer(int(q).to_bytes(4 * L, "little"), dtype="<u4")
+    return np.ascontiguousarray(q, dtype=np.uint32).reshape(L)
+
+
+class Server:
+    """Serving loop over a catalog of points (pm_server_*): submit host
+    operands for point i, wait for the result; a scheduler thread batches
+    whatever is waiting under the HBM and pinned-memory budgets.
+
+    points: list of (N, W, q); config keys: device_budget, pinned_budget,
+    max_batch, max_inflight, host_threads, col_tile (0 or absent: default)."""
+
+    def __init__(self, points, **config):
+        lib = load()
+        self.points = [(int(N), int(W)) for N, W, _ in points]
+        Ns = np.array([p[0] for p in points], dtype=np.uint32)
+        Ws = np.array([p[1] for p in points], dtype=np.uint32)
+        qc = np.ascontiguousarray(np.concatenate([_q_array(q, (W + 31) // 32) for _, W, q in points]))
+        cfg = ServerConfig(**{k: int(v) for k, v in config.items()})
+        h = _vp()
+        _check_server(lib.pm_server_create(len(points), _ptr(Ns), _ptr(Ws), _ptr(qc), ctypes.byref(cfg),
+                                           ctypes.byref(h)))
+        self._h = h
+        self._live = {}  # ticket -> (a, b, c) kept alive until done
+
+    def close(self):
+        if getattr(self, "_h", None):
+            load().pm_server_free(self._h)
+            self._h = None
+            self._live.clear()
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def __del__(self):
+        self.close()
+
+    def submit(self, point: int, a: np.ndarray, b: np.ndarray) -> int:
+        N, W = self.points[point]
+        L = (W + 31) // 32
+        a = _u32(a).reshape(N, L)
+        b = _u32(b).reshape(N, L)
+        c = np.empty((N, L), dtype=np.uint32)
+        t = ctypes.c_uint64()
+        _check_server(load().pm_server_submit(self._h, point, _ptr(a), _ptr(b), _ptr(c), ctypes.byref(t)))
+        self._live[t.value] = (a, b, c)
+        return t.value
+
+    def wait(self, ticket: int, timeout: float = 60.0):
+        """(c, latency_ms) once the request is done; None on timeout."""
+        ms = ctypes.c_double()
+        try:
+            rc = _check_server(load().pm_server_wait(self._h, ticket, timeout, ctypes.byref(ms)))
+        except PolymultError:
+            self._live.pop(ticket, None)
+            raise
+        if rc == 1:
+            return None
+        _, _, c = self._live.pop(ticket)
+        return c, ms.value
+
+    def stats(self, reset: bool = False) -> dict:
+        st = ServerStats()
+        _check_server(load().pm_server_stats_get(self._h, int(reset), ctypes.byref(st)))
+        return _as_dict(st)
+
+    def drive(self, rate: float, seconds: float, weights, seed: int = 1) -> dict:
+        """Open-loop Poisson load from a native driver; see pm_server_drive."""
+        w = (ctypes.c_double * len(self.points))(*[float(x) for x in weights])
+        rep = ServeReport()
+        _check_server(load().pm_server_drive(self._h, rate, seconds, w, seed, ctypes.byref(rep)))
+        return _as_dict(rep)

@@ -3,6 +3,7 @@ symbol include/polymult.h declares plus the reference's three C++ entry
 points, and its host-side point validation agrees with the oracle and with
 the reference's messages (solve.cu:873-913). No compute call touches a GPU.
 """
+import ctypes
 import re
 import subprocess
 from pathlib import Path
@@ -65,3 +66,26 @@ def test_q_limb_count_checked(O, pm):
     q = O.spec_q(64, 64)
     with pytest.raises(pm.PolymultError, match="q must have L limbs"):
         pm.point_info(64, 64, 2, O.q_limbs(q, 3))
+
+
+def test_struct_layouts_match_header(pm, tmp_path):
+    """The ctypes mirrors of the header's structs have the C compiler's sizes and offsets."""
+    import subprocess
+    structs = {"pm_info": pm.PmInfo, "pm_server_config": pm.ServerConfig, "pm_server_stats": pm.ServerStats,
+               "pm_serve_report": pm.ServeReport}
+    lines = ['#include <stdio.h>', '#include <stddef.h>', '#include "polymult.h"', "int main(void) {"]
+    for cname, py in structs.items():
+        lines.append('printf("%s %%zu\\n", sizeof(%s));' % (cname, cname))
+        for f, _ in py._fields_:
+            lines.append('printf("%s.%s %%zu\\n", offsetof(%s, %s));' % (cname, f, cname, f))
+    lines.append("return 0; }")
+    src = tmp_path / "layout.c"
+    src.write_text("\n".join(lines))
+    exe = tmp_path / "layout"
+    root = Path(__file__).resolve().parents[1]
+    subprocess.run(["gcc", "-std=c99", "-I", str(root / "include"), str(src), "-o", str(exe)], check=True)
+    got = dict(l.split() for l in subprocess.run([str(exe)], capture_output=True, text=True, check=True).stdout.split("\n") if l)
+    for cname, py in structs.items():
+        assert int(got[cname]) == ctypes.sizeof(py), cname
+        for f, _ in py._fields_:
+            assert int(got["%s.%s" % (cname, f)]) == getattr(py, f).offset, (cname, f)

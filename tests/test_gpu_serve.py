@@ -1,0 +1,98 @@
+"""Serving loop: requests from host buffers, continuous batching, admission by
+HBM footprint.
+
+A request's result must not depend on what it was batched with: each request
+is run alone, then all of them together, and the two results must be equal
+bit for bit and equal to the oracle (or, at the largest point, to the plain
+per-call path, itself checked against the oracle elsewhere).
+"""
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+MIX_POINTS = [(8192, 438), (32768, 868), (65536, 1747), (1024, 256)]
+
+
+def _catalog(O, points):
+    return [(N, W, O.spec_q(N, W)) for N, W in points]
+
+
+def test_serve_solo_equals_mix(O, pm):
+    cat = _catalog(O, MIX_POINTS)
+    kinds = ["mixed", "uniform", "max", "sparse", "small16"]
+    reqs = []
+    for i in range(12):
+        p = i % len(cat)
+        N, W, q = cat[p]
+        reqs.append((p, *O.operands(kinds[i % len(kinds)], N, W, q, 300 + i)))
+    with pm.Server(cat) as S:
+        solo = []
+        for p, a, b in reqs:
+            print("solo point", p, flush=True)
+            c, ms = S.wait(S.submit(p, a, b))
+            assert ms > 0
+            solo.append(c)
+        st = S.stats(reset=True)
+        assert st["completed"] == len(reqs) and st["batches"] == len(reqs)
+        print("mix of", len(reqs), flush=True)
+        tickets = [S.submit(p, a, b) for p, a, b in reqs]
+        mix = [S.wait(t)[0] for t in tickets]
+        st = S.stats()
+        print("stats", st, flush=True)
+        assert st["completed"] == len(reqs) and st["failed"] == 0
+        assert st["batches"] < len(reqs), "requests submitted together should share batches"
+    for i, (p, a, b) in enumerate(reqs):
+        assert np.array_equal(solo[i], mix[i]), "request %d differs alone and in the mix" % i
+    for i, (p, a, b) in enumerate(reqs):
+        N, W, q = cat[p]
+        if N <= 32768 or i == 2:
+            assert np.array_equal(mix[i], O.OraclePlan(N, W, q).multiply(a, b)), "request %d vs oracle" % i
+        else:
+            with pm.Multiplier(N, W, O.q_limbs(q, O.limbs_of(W))) as m:
+                assert np.array_equal(mix[i], m.run(a, b)), "request %d vs plain path" % i
+
+
+def test_serve_admission_under_tiny_budget(O, pm):
+    """A device budget that holds one N=32768 request at a time: every request
+    still completes, correctly, and the queue head waits for memory."""
+    cat = _catalog(O, [(32768, 868), (2048, 300), (65536, 1747)])
+    N, W, q = cat[0]
+    L, K = O.limbs_of(W), 57
+    one = 2 * N * L * 4 + 2 * K * N * 4
+    budget = int(one * 1.5)
+    reqs = [(0, *O.operands("uniform", N, W, q, 400 + i)) for i in range(5)]
+    N1, W1, q1 = cat[1]
+    reqs += [(1, *O.operands("mixed", N1, W1, q1, 410 + i)) for i in range(3)]
+    with pm.Server(cat, device_budget=budget, max_inflight=2) as S:
+        N3, W3, q3 = cat[2]
+        a3, b3 = O.operands("uniform", N3, W3, q3, 1)
+        with pytest.raises(pm.PolymultError, match="above the server's budget"):
+            S.submit(2, a3, b3)
+        tickets = [S.submit(p, a, b) for p, a, b in reqs]
+        outs = [S.wait(t)[0] for t in tickets]
+        st = S.stats()
+        print("stats", st, flush=True)
+        assert st["completed"] == len(reqs) and st["failed"] == 0
+        assert st["device_peak"] <= budget
+        assert st["deferred"] > 0
+    for (p, a, b), c in zip(reqs, outs):
+        N, W, q = cat[p]
+        assert np.array_equal(c, O.OraclePlan(N, W, q).multiply(a, b))
+
+
+def test_serve_poisson_drive_short(O, pm):
+    """One second of open-loop Poisson load on the serving mix: every request
+    completes, none fails, percentiles are ordered."""
+    cat = _catalog(O, MIX_POINTS[:3])
+    with pm.Server(cat) as S:
+        rep = S.drive(500.0, 1.0, [0.70, 0.25, 0.05], seed=7)
+        print("drive", rep, flush=True)
+        assert rep["requests"] > 300
+        assert rep["completed"] == rep["requests"] and rep["failed"] == 0
+        assert 0 < rep["p50_ms"] <= rep["p99_ms"] <= rep["max_ms"]
+        # the server stays usable and exact after the load
+        N, W, q = cat[0]
+        a, b = O.operands("mixed", N, W, q, 9)
+        c, _ = S.wait(S.submit(0, a, b))
+        assert np.array_equal(c, O.OraclePlan(N, W, q).multiply(a, b))
