@@ -125,7 +125,11 @@ def serving(pmlib, O, rates, seconds: float) -> dict:
         S.drive(200.0, 0.5, SERVE_MIX, seed=SEED)  # warm-up
         for r in rates:
             rep = S.drive(float(r), seconds, SERVE_MIX, seed=SEED + int(r))
-            out["rates"].append({k: (round(v, 3) if isinstance(v, float) else v) for k, v in rep.items()})
+            p50, p99 = rep.pop("point_p50_ms"), rep.pop("point_p99_ms")
+            row = {k: (round(v, 3) if isinstance(v, float) else v) for k, v in rep.items()}
+            row["per_point"] = {"%d/%d" % pt: {"p50_ms": round(p50[i], 3), "p99_ms": round(p99[i], 3)}
+                                for i, pt in enumerate(SERVE_POINTS)}
+            out["rates"].append(row)
     return out
 
 

@@ -137,14 +137,19 @@ typedef struct {
     uint64_t device_budget; /* HBM bytes for requests in flight (0: 4 GiB) */
     uint64_t pinned_budget; /* pinned staging bytes (0: 1 GiB) */
     uint32_t max_batch;     /* requests per batch (0: 64) */
-    uint32_t max_inflight;  /* batches in flight, at most 8 (0: 2) */
-    uint32_t host_threads;  /* staging threads incl. the scheduler (0: min(8, cpus - 2)) */
+    uint32_t max_inflight;  /* batches in flight, at most 8 (0: 3) */
+    uint32_t host_threads;  /* copy threads, half staging, half scattering (0: min(8, cpus - 2)) */
     int32_t col_tile;       /* column tile 8, 16 or 32 (0: 32) */
+    uint64_t max_batch_bytes; /* staged input bytes per batch, above one request (0: 64 MiB) */
 } pm_server_config;
 typedef struct {
     uint64_t submitted, completed, failed, batches;
     uint64_t deferred; /* times the queue head waited for memory */
     double mean_batch, p50_ms, p99_ms, max_ms, mean_ms;
+    /* per batch, mean ms: host staging, issuing copies and launches, issue
+     * to results back in pinned memory, scatter to the callers' buffers */
+    double stage_ms, issue_ms, gpu_ms, scatter_ms;
+    double sched_busy, completer_busy; /* fractions of wall time since reset */
     uint64_t device_budget, device_peak, pinned_budget, pinned_peak;
 } pm_server_stats;
 typedef struct {
@@ -153,6 +158,14 @@ typedef struct {
     double achieved_rate, p50_ms, p99_ms, max_ms, mean_ms, mean_batch;
     uint64_t batches, deferred, device_peak;
     double max_late_ms; /* worst delay of a submission behind its arrival time */
+    double stage_ms, issue_ms, gpu_ms, scatter_ms, sched_busy, completer_busy; /* as pm_server_stats */
+    /* latency split at admission into a batch: queue = arrival to admission,
+     * service = admission to result written */
+    double queue_p50_ms, queue_p99_ms, service_p50_ms, service_p99_ms;
+    double point_p50_ms[8], point_p99_ms[8]; /* per catalog point (first 8) */
+    double cpu_util;           /* process CPU seconds per wall second during the run */
+    uint64_t ctx_invol;        /* involuntary context switches of the process */
+    uint64_t driver_ctx_invol; /* ... of the driving thread */
 } pm_serve_report;
 int pm_server_create(uint32_t npoints, const uint32_t* N, const uint32_t* W, const uint32_t* q_concat,
                      const pm_server_config* cfg, pm_server** out);

@@ -223,6 +223,15 @@ void Runner::upload(hipStream_t s, bool pinned) {
               "upload tiles");
 }
 
+void Runner::reserve(size_t nprods, size_t ntasks) {
+    ensure(d_prods, prod_cap, nprods, "hipMalloc products");
+    ensure(d_tasks, task_cap, ntasks, "hipMalloc tiles");
+    ensure_pinned(h_prods, h_prod_cap, nprods);
+    ensure_pinned(h_tasks, h_task_cap, ntasks);
+    std::memset(static_cast<void*>(h_prods), 0, h_prod_cap * sizeof(BatchProduct));
+    std::memset(static_cast<void*>(h_tasks), 0, h_task_cap * sizeof(BatchTask));
+}
+
 hipError_t Runner::launch(hipStream_t s) const {
     const bool persistent = cfg.blocks_per_cu > 0;
     if (persistent) {

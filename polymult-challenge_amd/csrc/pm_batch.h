@@ -67,6 +67,9 @@ struct Runner {
     // copy; the mirrors are rewritten by the next upload, so the caller must
     // know the previous one has completed.
     void upload(hipStream_t s, bool pinned = false);
+    // Sizes the device buffers and the pinned mirrors (touched, so a later
+    // upload takes no page faults) for up to nprods products and ntasks tiles.
+    void reserve(size_t nprods, size_t ntasks);
     // Enqueues the phases (and the counter reset of the persistent mode).
     hipError_t launch(hipStream_t s) const;
     ~Runner();

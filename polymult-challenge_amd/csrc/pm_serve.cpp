@@ -11,11 +11,15 @@
 // once, so one batch's transfers overlap another's compute. Latency is
 // measured from a request's arrival to the end of its scatter.
 #include <hip/hip_runtime.h>
+#include <sys/resource.h>
 
 #include <algorithm>
 #include <chrono>
 #include <cmath>
 #include <condition_variable>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
 #include <deque>
 #include <map>
 #include <memory>
@@ -101,7 +105,7 @@ struct Request {
     const uint32_t* a = nullptr;
     const uint32_t* b = nullptr;
     uint32_t* c = nullptr;
-    double t_arrive = 0, t_done = 0;
+    double t_arrive = 0, t_take = 0, t_done = 0;
     size_t dev_off = 0, pin_off = 0;
     int state = 0;  // 0 queued, 1 in a batch, 2 done, -1 failed
     std::string error;
@@ -130,43 +134,69 @@ struct Server {
     unsigned char* h_arena = nullptr;
     Arena dev_arena, pin_arena;
     size_t max_batch = 64;
+    size_t max_batch_bytes = 0;
     hipStream_t s_in = nullptr, s_comp = nullptr, s_out = nullptr;
-    std::unique_ptr<Team> team;
+    std::unique_ptr<Team> stage_team, scatter_team;
 
     struct Slot {
         Runner run;
         std::vector<Request*> reqs;
         hipEvent_t ev_in = nullptr, ev_comp = nullptr, ev_out = nullptr;
+        double t_take = 0, t_staged = 0, t_issued = 0;
     };
     std::vector<std::unique_ptr<Slot>> slots;
-    std::deque<Slot*> inflight;  // oldest first
-    std::vector<Slot*> idle;
 
+    // Everything below is guarded by m.
     std::mutex m;
-    std::condition_variable cv_work, cv_done;
+    std::condition_variable cv_work;    // scheduler: work_gen moved
+    std::condition_variable cv_flight;  // completer: a batch was issued, or the scheduler ended
+    std::condition_variable cv_done;    // waiters: a request finished
+    std::deque<Slot*> inflight;         // issued batches, oldest first
+    std::vector<Slot*> idle;
     std::deque<Request*> queue;
     std::unordered_map<uint64_t, std::unique_ptr<Request>> reqs;
     uint64_t next_id = 1;
-    bool stop = false;
+    uint64_t work_gen = 0;  // bumped by every submission, release and stop
+    bool stop = false, sched_exit = false;
     std::string broken;
     uint64_t last_deferred = 0;
 
     // statistics since the last reset
     std::vector<double> lat_ms;
     uint64_t n_submitted = 0, n_completed = 0, n_failed = 0, n_batches = 0, n_batched = 0, n_deferred = 0;
+    double s_stage = 0, s_issue = 0, s_gpu = 0, s_scatter = 0, t_reset = 0;
 
-    std::thread sched;
+    std::thread sched, completer;
+
+    // PM_SERVE_TRACE=<prefix>: per-batch and per-request timelines written
+    // as <prefix>.batches.csv and <prefix>.requests.csv when the server ends.
+    std::string trace_path;
+    struct BatchRec {
+        double take, staged, issued, seen, done;
+        uint32_t nreq;
+        uint64_t bytes;
+    };
+    struct ReqRec {
+        double arrive, take, done;
+        uint32_t point;
+    };
+    std::vector<BatchRec> trace_b;
+    std::vector<ReqRec> trace_r;
+    void write_trace();
 
     void create(uint32_t npoints, const uint32_t* N, const uint32_t* W, const uint32_t* q, const pm_server_config* c);
     ~Server();
 
     uint64_t submit(uint32_t point, const uint32_t* a, const uint32_t* b, uint32_t* c, double t_arrive);
-    int wait(uint64_t id, double timeout_s, double* latency_ms);
+    // 0 done, 1 not yet; latency from arrival to done, queue_ms from arrival
+    // to admission into a batch
+    int wait(uint64_t id, double timeout_s, double* latency_ms, double* queue_ms = nullptr);
     void stats(pm_server_stats* out, bool reset);
 
   private:
-    void loop();
-    std::vector<Request*> take();
+    void schedule_loop();
+    void complete_loop();
+    std::vector<Request*> take_locked();
     void launch(Slot& S);
     void retire(Slot& S, hipError_t status);
     void fail_batch(Slot& S, const std::string& why);
@@ -179,7 +209,8 @@ void Server::create(uint32_t npoints, const uint32_t* N, const uint32_t* W, cons
     const size_t dev_budget = C.device_budget ? C.device_budget : (size_t)4 << 30;
     const size_t pin_budget = C.pinned_budget ? C.pinned_budget : (size_t)1 << 30;
     max_batch = C.max_batch ? C.max_batch : 64;
-    const int ninflight = C.max_inflight ? (int)C.max_inflight : 2;
+    max_batch_bytes = C.max_batch_bytes ? C.max_batch_bytes : (size_t)64 << 20;
+    const int ninflight = C.max_inflight ? (int)C.max_inflight : 3;
     if (ninflight > 8) throw std::runtime_error("max_inflight above 8");
     cfg.col_tc = C.col_tile ? C.col_tile : 32;
     if (cfg.col_tc != 8 && cfg.col_tc != 16 && cfg.col_tc != 32) throw std::runtime_error("col_tile must be 8, 16 or 32");
@@ -194,6 +225,9 @@ void Server::create(uint32_t npoints, const uint32_t* N, const uint32_t* W, cons
     hip_check(hipMalloc(&d_arena, dev_budget), "hipMalloc request arena");
     hip_check(hipHostMalloc(reinterpret_cast<void**>(&h_arena), pin_budget, hipHostMallocDefault),
               "hipHostMalloc staging arena");
+    // The first CPU touch of a pinned page faults; do it here, not while a
+    // batch is staging.
+    std::memset(h_arena, 0, pin_budget);
     dev_arena.reset(dev_budget);
     pin_arena.reset(pin_budget);
     hip_check(hipStreamCreateWithFlags(&s_in, hipStreamNonBlocking), "stream");
@@ -208,27 +242,42 @@ void Server::create(uint32_t npoints, const uint32_t* N, const uint32_t* W, cons
         idle.push_back(S.get());
         slots.push_back(std::move(S));
     }
-    // Every point must plan on this device (LDS) before a request can arrive.
+    // Every point must plan on this device (LDS) before a request can arrive;
+    // the largest tile count of one product sizes the slots' tile buffers.
+    size_t max_tiles = 0;
     {
         Runner& R = slots[0]->run;
-        R.prods.assign(npoints, BatchProduct{});
-        for (uint32_t i = 0; i < npoints; ++i) R.prods[i].D = cat.dev[i];
-        R.plan(cfg);
+        for (uint32_t i = 0; i < npoints; ++i) {
+            R.prods.assign(1, BatchProduct{});
+            R.prods[0].D = cat.dev[i];
+            R.plan(cfg);
+            max_tiles = std::max(max_tiles, R.tasks.size());
+        }
         R.prods.clear();
         R.tasks.clear();
     }
-    const int threads = C.host_threads ? (int)C.host_threads : std::max(1, std::min(8, host_cpus() - 2));
-    team = std::make_unique<Team>(threads - 1);
-    sched = std::thread([this] { loop(); });
+    for (auto& S : slots) S->run.reserve(max_batch, max_batch * max_tiles);
+    // host threads: half stage inputs (with the scheduler), half scatter
+    // results (with the completer)
+    const int threads = C.host_threads ? (int)C.host_threads : std::max(2, std::min(8, host_cpus() - 2));
+    stage_team = std::make_unique<Team>(std::max(0, threads / 2 - 1));
+    scatter_team = std::make_unique<Team>(std::max(0, threads - threads / 2 - 1));
+    t_reset = now_s();
+    if (const char* tp = std::getenv("PM_SERVE_TRACE")) trace_path = tp;
+    sched = std::thread([this] { schedule_loop(); });
+    completer = std::thread([this] { complete_loop(); });
 }
 
 Server::~Server() {
     {
         std::lock_guard<std::mutex> lk(m);
         stop = true;
+        ++work_gen;
     }
     cv_work.notify_all();
     if (sched.joinable()) sched.join();
+    if (completer.joinable()) completer.join();
+    if (!trace_path.empty()) write_trace();
     for (hipStream_t s : {s_in, s_comp, s_out})
         if (s) (void)hipStreamSynchronize(s);
     for (auto& S : slots)
@@ -267,12 +316,13 @@ uint64_t Server::submit(uint32_t point, const uint32_t* a, const uint32_t* b, ui
         queue.push_back(r.get());
         reqs.emplace(id, std::move(r));
         ++n_submitted;
+        ++work_gen;
     }
     cv_work.notify_one();
     return id;
 }
 
-int Server::wait(uint64_t id, double timeout_s, double* latency_ms) {
+int Server::wait(uint64_t id, double timeout_s, double* latency_ms, double* queue_ms) {
     std::unique_lock<std::mutex> lk(m);
     auto it = reqs.find(id);
     if (it == reqs.end()) throw std::runtime_error("unknown ticket");
@@ -283,6 +333,7 @@ int Server::wait(uint64_t id, double timeout_s, double* latency_ms) {
         if (!cv_done.wait_for(lk, std::chrono::duration<double>(timeout_s), ready)) return 1;
     }
     if (latency_ms) *latency_ms = 1e3 * (r->t_done - r->t_arrive);
+    if (queue_ms) *queue_ms = 1e3 * (r->t_take - r->t_arrive);
     if (r->state < 0) {
         const std::string e = r->error;
         reqs.erase(it);
@@ -311,6 +362,15 @@ void Server::stats(pm_server_stats* out, bool reset) {
         for (double x : v) s += x;
         out->mean_ms = s / v.size();
     }
+    if (n_batches) {
+        out->stage_ms = 1e3 * s_stage / n_batches;
+        out->issue_ms = 1e3 * s_issue / n_batches;
+        out->gpu_ms = 1e3 * s_gpu / n_batches;
+        out->scatter_ms = 1e3 * s_scatter / n_batches;
+    }
+    const double wall = std::max(1e-9, now_s() - t_reset);
+    out->sched_busy = (s_stage + s_issue) / wall;
+    out->completer_busy = s_scatter / wall;
     out->device_budget = dev_arena.size();
     out->device_peak = dev_arena.peak();
     out->pinned_budget = pin_arena.size();
@@ -318,6 +378,8 @@ void Server::stats(pm_server_stats* out, bool reset) {
     if (reset) {
         lat_ms.clear();
         n_submitted = n_completed = n_failed = n_batches = n_batched = n_deferred = 0;
+        s_stage = s_issue = s_gpu = s_scatter = 0;
+        t_reset = now_s();
         dev_arena.clear_peak();
         pin_arena.clear_peak();
     }
@@ -325,12 +387,14 @@ void Server::stats(pm_server_stats* out, bool reset) {
 
 // Admits requests from the head of the queue while they fit both budgets.
 // Strict arrival order: a head that does not fit holds back the ones behind
-// it (no starvation of large requests). Called with no lock held.
-std::vector<Request*> Server::take() {
+// it (no starvation of large requests). Called with m held.
+std::vector<Request*> Server::take_locked() {
     std::vector<Request*> batch;
-    std::lock_guard<std::mutex> lk(m);
+    size_t bytes = 0;
     while (!queue.empty() && batch.size() < max_batch) {
         Request* r = queue.front();
+        // a large backlog leaves as several pipelined batches, not one
+        if (!batch.empty() && bytes + pin_need[r->point] > max_batch_bytes) break;
         size_t doff = 0, poff = 0;
         if (!dev_arena.alloc(dev_need[r->point], doff)) {
             if (r->id != last_deferred) ++n_deferred, last_deferred = r->id;
@@ -344,8 +408,10 @@ std::vector<Request*> Server::take() {
         r->dev_off = doff;
         r->pin_off = poff;
         r->state = 1;
+        r->t_take = now_s();
         queue.pop_front();
         batch.push_back(r);
+        bytes += pin_need[r->point];
     }
     return batch;
 }
@@ -359,7 +425,8 @@ void Server::launch(Slot& S) {
         add_pieces(pieces, h_arena + r->pin_off, r->a, n);
         add_pieces(pieces, h_arena + r->pin_off + io_bytes[r->point], r->b, n);
     }
-    team->run((int)pieces.size(), [&](int i) { copy_stream(pieces[i].dst, pieces[i].src, pieces[i].n); });
+    stage_team->run((int)pieces.size(), [&](int i) { copy_stream(pieces[i].dst, pieces[i].src, pieces[i].n); });
+    S.t_staged = now_s();
     for (Request* r : S.reqs)
         hip_check(hipMemcpyAsync(dev + r->dev_off, h_arena + r->pin_off,
                                  io_bytes[r->point] + cat.io_words(r->point) * 4, hipMemcpyHostToDevice, s_in),
@@ -391,19 +458,34 @@ void Server::launch(Slot& S) {
                                  hipMemcpyDeviceToHost, s_out),
                   "D2H");
     hip_check(hipEventRecord(S.ev_out, s_out), "record");
+    S.t_issued = now_s();
 }
 
+// Called by the completer once the batch's last D2H has finished (or failed).
 void Server::retire(Slot& S, hipError_t status) {
     if (status != hipSuccess) {
         fail_batch(S, std::string("batch failed: ") + hipGetErrorString(status));
         return;
     }
+    const double t_seen = now_s();
     std::vector<Piece> pieces;
     for (Request* r : S.reqs) add_pieces(pieces, r->c, h_arena + r->pin_off, cat.io_words(r->point) * 4);
-    team->run((int)pieces.size(), [&](int i) { copy_stream(pieces[i].dst, pieces[i].src, pieces[i].n); });
+    scatter_team->run((int)pieces.size(), [&](int i) { copy_stream(pieces[i].dst, pieces[i].src, pieces[i].n); });
     const double t = now_s();
     {
         std::lock_guard<std::mutex> lk(m);
+        s_stage += S.t_staged - S.t_take;
+        s_issue += S.t_issued - S.t_staged;
+        s_gpu += t_seen - S.t_issued;
+        s_scatter += t - t_seen;
+        if (!trace_path.empty()) {
+            uint64_t bytes = 0;
+            for (Request* r : S.reqs) {
+                bytes += 12 * cat.io_words(r->point);
+                trace_r.push_back({r->t_arrive, r->t_take, t, r->point});
+            }
+            trace_b.push_back({S.t_take, S.t_staged, S.t_issued, t_seen, t, (uint32_t)S.reqs.size(), bytes});
+        }
         for (Request* r : S.reqs) {
             r->t_done = t;
             r->state = 2;
@@ -414,9 +496,11 @@ void Server::retire(Slot& S, hipError_t status) {
         n_completed += S.reqs.size();
         n_batches += 1;
         n_batched += S.reqs.size();
+        ++work_gen;
     }
     S.reqs.clear();
     cv_done.notify_all();
+    cv_work.notify_one();
 }
 
 void Server::fail_batch(Slot& S, const std::string& why) {
@@ -440,51 +524,96 @@ void Server::fail_batch(Slot& S, const std::string& why) {
             ++n_failed;
         }
         queue.clear();
+        ++work_gen;
     }
     S.reqs.clear();
     cv_done.notify_all();
+    cv_work.notify_one();
 }
 
-void Server::loop() {
+void Server::write_trace() {
+    if (FILE* f = std::fopen((trace_path + ".batches.csv").c_str(), "w")) {
+        std::fprintf(f, "take,staged,issued,seen,done,nreq,bytes\n");
+        for (const BatchRec& b : trace_b)
+            std::fprintf(f, "%.6f,%.6f,%.6f,%.6f,%.6f,%u,%llu\n", b.take, b.staged, b.issued, b.seen, b.done, b.nreq,
+                         (unsigned long long)b.bytes);
+        std::fclose(f);
+    }
+    if (FILE* f = std::fopen((trace_path + ".requests.csv").c_str(), "w")) {
+        std::fprintf(f, "arrive,take,done,point\n");
+        for (const ReqRec& r : trace_r) std::fprintf(f, "%.6f,%.6f,%.6f,%u\n", r.arrive, r.take, r.done, r.point);
+        std::fclose(f);
+    }
+}
+
+// Forms and issues batches: whenever a slot is idle and the queue head fits
+// the budgets, takes everything admissible, stages it and enqueues the
+// batch's copies and launches; otherwise sleeps until a submission or a
+// release changes that.
+void Server::schedule_loop() {
     for (;;) {
-        // retire every finished batch, oldest first
-        while (!inflight.empty()) {
-            Slot* S = inflight.front();
-            const hipError_t q = hipEventQuery(S->ev_out);
-            if (q == hipErrorNotReady) break;
-            inflight.pop_front();
-            retire(*S, q);
-            idle.push_back(S);
-        }
-        // start a batch if a slot is free and something admissible waits
-        if (!idle.empty()) {
-            std::vector<Request*> batch = take();
-            if (!batch.empty()) {
-                Slot* S = idle.back();
-                idle.pop_back();
-                S->reqs = std::move(batch);
-                try {
-                    launch(*S);
-                    inflight.push_back(S);
-                } catch (const std::exception& e) {
-                    // drain what was already issued before reusing anything
-                    for (hipStream_t s : {s_in, s_comp, s_out}) (void)hipStreamSynchronize(s);
-                    fail_batch(*S, e.what());
-                    idle.push_back(S);
-                }
-                continue;
-            }
-        }
-        if (inflight.empty()) {
+        Slot* S = nullptr;
+        std::vector<Request*> batch;
+        {
             std::unique_lock<std::mutex> lk(m);
-            if (queue.empty()) {
-                if (stop) return;
-                cv_work.wait(lk, [&] { return stop || !queue.empty(); });
+            for (;;) {
+                if (stop && queue.empty()) {
+                    sched_exit = true;
+                    lk.unlock();
+                    cv_flight.notify_all();
+                    return;
+                }
+                if (!queue.empty() && !idle.empty()) {
+                    batch = take_locked();
+                    if (!batch.empty()) break;
+                }
+                const uint64_t seen = work_gen;
+                cv_work.wait(lk, [&] { return work_gen != seen; });
             }
+            S = idle.back();
+            idle.pop_back();
+        }
+        S->reqs = std::move(batch);
+        S->t_take = now_s();
+        try {
+            launch(*S);
+        } catch (const std::exception& e) {
+            // drain what was already issued before reusing anything
+            for (hipStream_t s : {s_in, s_comp, s_out}) (void)hipStreamSynchronize(s);
+            fail_batch(*S, e.what());
+            std::lock_guard<std::mutex> lk(m);
+            idle.push_back(S);
             continue;
         }
-        // work in flight and nothing to start: poll again shortly
-        for (int i = 0; i < 64; ++i) relax();
+        {
+            std::lock_guard<std::mutex> lk(m);
+            inflight.push_back(S);
+        }
+        cv_flight.notify_one();
+    }
+}
+
+// Retires issued batches in order: waits for the last D2H, scatters the
+// results, releases the memory and hands the slot back.
+void Server::complete_loop() {
+    for (;;) {
+        Slot* S = nullptr;
+        {
+            std::unique_lock<std::mutex> lk(m);
+            cv_flight.wait(lk, [&] { return !inflight.empty() || sched_exit; });
+            if (inflight.empty()) return;
+            S = inflight.front();
+        }
+        hipError_t q;
+        while ((q = hipEventQuery(S->ev_out)) == hipErrorNotReady) relax();
+        retire(*S, q);
+        {
+            std::lock_guard<std::mutex> lk(m);
+            inflight.pop_front();
+            idle.push_back(S);
+            ++work_gen;
+        }
+        cv_work.notify_one();
     }
 }
 
@@ -522,7 +651,14 @@ void drive(Server& S, double rate, double seconds, const double* weights, uint64
     for (size_t i = 0; i < np; ++i)
         if (w[i] > 0)
             for (int k = 0; k < 4; ++k) ops[i].push_back(random_operand(g, S.cat.plans[i].N, S.cat.plans[i].W));
-    std::vector<std::vector<std::unique_ptr<std::vector<uint32_t>>>> pool(np);  // free result buffers
+    // Result buffers, allocated and touched before the clock starts (a first
+    // touch inside the run would charge page faults to the latencies); the
+    // pool grows only if more requests of a point are outstanding at once.
+    std::vector<std::vector<std::unique_ptr<std::vector<uint32_t>>>> pool(np);
+    for (size_t i = 0; i < np; ++i)
+        if (w[i] > 0)
+            for (int k = (int)std::ceil(64.0 * w[i] / wsum) + 2; k > 0; --k)
+                pool[i].push_back(std::make_unique<std::vector<uint32_t>>(S.cat.io_words(i), 1u));
     std::discrete_distribution<int> pick(w.begin(), w.end());
     std::exponential_distribution<double> gap(rate);
 
@@ -532,27 +668,33 @@ void drive(Server& S, double rate, double seconds, const double* weights, uint64
         std::unique_ptr<std::vector<uint32_t>> buf;
     };
     std::deque<Out> outstanding;
-    std::vector<double> lat;
+    std::vector<double> lat, qlat, slat;
+    std::vector<std::vector<double>> plat(np);
     uint64_t failed = 0, nreq = 0;
     double max_late = 0;
+    // Requests finish in batch order, so reaping stops at the first one not
+    // done (or waits for each, at the end).
     auto reap = [&](bool block) {
         for (auto it = outstanding.begin(); it != outstanding.end();) {
-            double ms = 0;
+            double ms = 0, qms = 0;
             int rc;
             try {
-                rc = S.wait(it->id, block ? 60.0 : 0.0, &ms);
+                rc = S.wait(it->id, block ? 60.0 : 0.0, &ms, &qms);
             } catch (const std::exception&) {
                 rc = -1;
             }
             if (rc == 1) {
                 if (block) throw std::runtime_error("request did not complete within 60 s");
-                ++it;
-                continue;
+                break;
             }
-            if (rc == 0)
+            if (rc == 0) {
                 lat.push_back(ms);
-            else
+                qlat.push_back(qms);
+                slat.push_back(ms - qms);
+                plat[it->point].push_back(ms);
+            } else {
                 ++failed;
+            }
             pool[it->point].push_back(std::move(it->buf));
             it = outstanding.erase(it);
         }
@@ -560,6 +702,9 @@ void drive(Server& S, double rate, double seconds, const double* weights, uint64
 
     pm_server_stats st0;
     S.stats(&st0, true);
+    rusage ru0{}, rt0{};
+    getrusage(RUSAGE_SELF, &ru0);
+    getrusage(RUSAGE_THREAD, &rt0);
     const double t0 = now_s() + 0.02;
     double t = 0;
     for (;;) {
@@ -585,6 +730,10 @@ void drive(Server& S, double rate, double seconds, const double* weights, uint64
     }
     reap(true);
     const double t_end = now_s();
+    rusage ru1{}, rt1{};
+    getrusage(RUSAGE_SELF, &ru1);
+    getrusage(RUSAGE_THREAD, &rt1);
+    auto secs = [](const timeval& a, const timeval& b) { return (b.tv_sec - a.tv_sec) + 1e-6 * (b.tv_usec - a.tv_usec); };
     pm_server_stats st;
     S.stats(&st, true);
     *out = pm_serve_report{};
@@ -596,6 +745,18 @@ void drive(Server& S, double rate, double seconds, const double* weights, uint64
     out->achieved_rate = lat.size() / std::max(1e-9, t_end - t0);
     if (!lat.empty()) {
         std::sort(lat.begin(), lat.end());
+        std::sort(qlat.begin(), qlat.end());
+        std::sort(slat.begin(), slat.end());
+        out->queue_p50_ms = percentile(qlat, 0.50);
+        out->queue_p99_ms = percentile(qlat, 0.99);
+        out->service_p50_ms = percentile(slat, 0.50);
+        out->service_p99_ms = percentile(slat, 0.99);
+        for (size_t i = 0; i < np && i < 8; ++i) {
+            if (plat[i].empty()) continue;
+            std::sort(plat[i].begin(), plat[i].end());
+            out->point_p50_ms[i] = percentile(plat[i], 0.50);
+            out->point_p99_ms[i] = percentile(plat[i], 0.99);
+        }
         out->p50_ms = percentile(lat, 0.50);
         out->p99_ms = percentile(lat, 0.99);
         out->max_ms = lat.back();
@@ -608,6 +769,15 @@ void drive(Server& S, double rate, double seconds, const double* weights, uint64
     out->deferred = st.deferred;
     out->device_peak = st.device_peak;
     out->max_late_ms = 1e3 * max_late;
+    out->cpu_util = (secs(ru0.ru_utime, ru1.ru_utime) + secs(ru0.ru_stime, ru1.ru_stime)) / std::max(1e-9, t_end - t0);
+    out->ctx_invol = (uint64_t)(ru1.ru_nivcsw - ru0.ru_nivcsw);
+    out->driver_ctx_invol = (uint64_t)(rt1.ru_nivcsw - rt0.ru_nivcsw);
+    out->stage_ms = st.stage_ms;
+    out->issue_ms = st.issue_ms;
+    out->gpu_ms = st.gpu_ms;
+    out->scatter_ms = st.scatter_ms;
+    out->sched_busy = st.sched_busy;
+    out->completer_busy = st.completer_busy;
 }
 
 }  // namespace

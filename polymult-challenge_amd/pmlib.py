@@ -49,12 +49,15 @@ class PmInfo(ctypes.Structure):
 class ServerConfig(ctypes.Structure):
     _fields_ = [("device_budget", ctypes.c_uint64), ("pinned_budget", ctypes.c_uint64),
                 ("max_batch", ctypes.c_uint32), ("max_inflight", ctypes.c_uint32),
-                ("host_threads", ctypes.c_uint32), ("col_tile", ctypes.c_int32)]
+                ("host_threads", ctypes.c_uint32), ("col_tile", ctypes.c_int32),
+                ("max_batch_bytes", ctypes.c_uint64)]
 
 
 class ServerStats(ctypes.Structure):
     _fields_ = [(n, ctypes.c_uint64) for n in ("submitted", "completed", "failed", "batches", "deferred")] + \
-               [(n, ctypes.c_double) for n in ("mean_batch", "p50_ms", "p99_ms", "max_ms", "mean_ms")] + \
+               [(n, ctypes.c_double) for n in ("mean_batch", "p50_ms", "p99_ms", "max_ms", "mean_ms", "stage_ms",
+                                               "issue_ms", "gpu_ms", "scatter_ms", "sched_busy",
+                                               "completer_busy")] + \
                [(n, ctypes.c_uint64) for n in ("device_budget", "device_peak", "pinned_budget", "pinned_peak")]
 
 
@@ -64,11 +67,19 @@ class ServeReport(ctypes.Structure):
                [(n, ctypes.c_double) for n in ("achieved_rate", "p50_ms", "p99_ms", "max_ms", "mean_ms",
                                                "mean_batch")] + \
                [(n, ctypes.c_uint64) for n in ("batches", "deferred", "device_peak")] + \
-               [("max_late_ms", ctypes.c_double)]
+               [(n, ctypes.c_double) for n in ("max_late_ms", "stage_ms", "issue_ms", "gpu_ms", "scatter_ms",
+                                               "sched_busy", "completer_busy", "queue_p50_ms", "queue_p99_ms",
+                                               "service_p50_ms", "service_p99_ms")] + \
+               [("point_p50_ms", ctypes.c_double * 8), ("point_p99_ms", ctypes.c_double * 8),
+                ("cpu_util", ctypes.c_double), ("ctx_invol", ctypes.c_uint64), ("driver_ctx_invol", ctypes.c_uint64)]
 
 
 def _as_dict(st: ctypes.Structure) -> dict:
-    return {name: getattr(st, name) for name, _ in st._fields_}
+    out = {}
+    for name, _ in st._fields_:
+        v = getattr(st, name)
+        out[name] = list(v) if isinstance(v, ctypes.Array) else v
+    return out
 
 
 _LIB = None
@@ -360,7 +371,8 @@ class Server:
     whatever is waiting under the HBM and pinned-memory budgets.
 
     points: list of (N, W, q); config keys: device_budget, pinned_budget,
-    max_batch, max_inflight, host_threads, col_tile (0 or absent: default)."""
+    max_batch, max_inflight, host_threads, col_tile, max_batch_bytes (0 or
+    absent: default)."""
 
     def __init__(self, points, **config):
         lib = load()
