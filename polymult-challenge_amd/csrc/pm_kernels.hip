@@ -14,6 +14,17 @@
 // prime, rows contiguous, columns strided. Every NTT kernel keeps 16 values
 // per thread in registers and does four radix-2 stages there; the two halves
 // of an 8-stage pass meet through one LDS transpose.
+//
+// What each kernel computes is what the reference computes at the same step
+// (the values are identical, the method is not):
+//   k_rns  = k_decompose (solve.cu:147-202): a_n mod p_j
+//   k_col  = k_col<INV> (solve.cu:254-284): NTT stages 0 .. w1-1 per column
+//   k_row  = k_row_fwd / k_row_fused (solve.cu:340-378): stages w1 .. logN-1,
+//            the pointwise product (Montgomery, 2^-32) and the inverse rows
+//   k_crt  = k_crt (solve.cu:380-501): y_j, v, z = sum y_j Q_j + T[v], fold, -q
+// The twiddle of CT stage s, block i is psi^bitrev(2^s + i) and the inverse
+// uses the mirrored entry; the table entries are the reference's tw(k)
+// (solve.cu:947-953), here as Shoup pairs rather than Montgomery forms.
 #include "pm_kernels.h"
 
 #include "pm_arith.h"
@@ -625,7 +636,7 @@ template <int PH>
 __device__ __forceinline__ void batch_task(const BatchProduct& P, u32 x, u32 y, u32* lds, int tid) {
     const DevPlan& D = P.D;
     if constexpr (PH == PH_RNS) {
-        rns_tile<kBatchRnsTile / 32>(D, y ? P.b : P.a, y ? P.R1 : P.R0, x, D.N, 1, lds, tid, kBatchThreads);
+        rns_tile<kBatchRnsTile / 32>(D, y ? P.b : P.a, y ? P.R1 : P.R0, x, D.N, 1, lds, tid, kBatchThreads[PH]);
     } else if constexpr (PH == PH_COLF) {
         col_tile_nb<false>(D.w1, D, (y >> 16) ? P.R1 : P.R0, (int)(y & 0xFFFF), (int)x, P.col_tc, lds, tid);
     } else if constexpr (PH == PH_ROW) {
@@ -634,7 +645,7 @@ __device__ __forceinline__ void batch_task(const BatchProduct& P, u32 x, u32 y, 
         col_tile_nb<true>(D.w1, D, P.R1, (int)y, (int)x, P.col_tc, lds, tid);
     } else {
         crt_tile<kBatchCrtTile / 32>(D, P.R1, P.c, D.G, x, D.N, P.crt_ys, reinterpret_cast<unsigned char*>(lds), tid,
-                                     kBatchThreads);
+                                     kBatchThreads[PH]);
     }
 }
 
@@ -643,7 +654,7 @@ __global__ void k_batch_reset(u32* ctl, u32 n) {
 }
 
 template <int PH>
-__global__ void __launch_bounds__(kBatchThreads) k_batch_phase(const BatchProduct* __restrict__ prods,
+__global__ void __launch_bounds__(kBatchThreads[PH]) k_batch_phase(const BatchProduct* __restrict__ prods,
                                                                const BatchTask* __restrict__ tasks, u32 ntasks,
                                                                u32* ctr) {
     extern __shared__ __align__(16) u32 lds[];
@@ -790,7 +801,7 @@ hipError_t launch_batch_reset(u32* ctl, u32 nctl, hipStream_t s) {
 hipError_t launch_batch_phase(int phase, const BatchProduct* prods, const BatchTask* tasks, u32 ntasks, u32* ctr,
                               int grid, size_t lds, hipStream_t s) {
     if (ntasks == 0) return hipSuccess;
-    const dim3 g(ctr ? (u32)grid : ntasks), b(kBatchThreads);
+    const dim3 g(ctr ? (u32)grid : ntasks), b(kBatchThreads[phase]);
     switch (phase) {
 #define PM_PH(n) \
     case n: hipLaunchKernelGGL((k_batch_phase<n>), g, b, lds, s, prods, tasks, ntasks, ctr); break;
