@@ -158,8 +158,8 @@ void Runner::plan(const BatchConfig& c) {
         D.w2 = logN - D.w1;
         const uint32_t nrows = 1u << D.w1, ncols = 1u << D.w2;
         const int Tc = line_threads(D.w1), Tr = line_threads(D.w2);
-        B.col_tc = (int)std::min<uint32_t>({(uint32_t)c.col_tc, ncols, (uint32_t)(kBatchThreads / Tc)});
-        B.row_rpb = (int)std::min<uint32_t>(nrows, (uint32_t)(kBatchThreads / Tr));
+        B.col_tc = (int)std::min<uint32_t>({(uint32_t)c.col_tc, ncols, (uint32_t)(kBatchThreads[PH_COLF] / Tc)});
+        B.row_rpb = (int)std::min<uint32_t>(nrows, (uint32_t)(kBatchThreads[PH_ROW] / Tr));
         B.crt_ys = 32 * D.crt_ks + 16;
         lds[PH_RNS] = std::max(lds[PH_RNS], (size_t)kBatchRnsTile * 4 * D.L + 32);
         lds[PH_CRT] = std::max(lds[PH_CRT], (size_t)kBatchCrtTile * B.crt_ys + (size_t)kBatchCrtTile * (D.L + 3) * 8);

@@ -89,7 +89,9 @@ struct BatchTask {
     uint32_t prod, x, y;
 };
 
-constexpr int kBatchThreads = 256;
+// Block size of each phase kernel (as the plain kernels: 8 waves for RNS
+// and the column passes, 4 for the rows and the CRT).
+constexpr int kBatchThreads[PH_COUNT] = {512, 512, 256, 512, 256};
 constexpr int kBatchRnsTile = 128;
 constexpr int kBatchCrtTile = 64;
 
