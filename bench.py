@@ -104,8 +104,27 @@ def batched(pmlib, O, count: int, points, seed: int, check: int, iters: int) -> 
         n, w, q = prods[i]
         exact &= bool(np.array_equal(dc[i].download((n, O.limbs_of(w))), O.OraclePlan(n, w, q).multiply(*ops[i])))
     ms = B.time(da, db, dc, iters)
+    B.close()
+    # the plain per-call path on the same products: one device-resident
+    # multiply per product, back to back (HIP events on the state's stream)
+    plain_ms = {}
+    for (n, w, q), a_, b_, c_ in zip(prods, da, db, dc):
+        if (n, w) not in plain_ms:
+            with pmlib.Multiplier(n, w, O.q_limbs(q, O.limbs_of(w))) as m:
+                for _ in range(3):
+                    m.run_device(a_, b_, c_)
+                m.sync()
+                t = time.perf_counter()
+                for _ in range(10):
+                    m.run_device(a_, b_, c_)
+                m.sync()
+                plain_ms[(n, w)] = 1e3 * (time.perf_counter() - t) / 10
+    plain_total = sum(plain_ms[(n, w)] for n, w, _ in prods)
     io = sum(12 * n * O.limbs_of(w) for n, w in points)
     return {"products": count, "ms_per_launch": round(ms, 4), "mults_per_s": round(count / (ms * 1e-3), 1),
+            "plain_per_call_ms": round(plain_total, 4),
+            "plain_per_call_mults_per_s": round(count / (plain_total * 1e-3), 1),
+            "speedup_vs_plain": round(plain_total / ms, 3),
             "io_bytes": io, "bit_exact_checked": min(check, count), "bit_exact": exact, "autotune": tuned}
 
 

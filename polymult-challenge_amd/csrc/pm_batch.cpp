@@ -170,43 +170,68 @@ void Runner::plan(const BatchConfig& c) {
     }
     for (size_t l : lds)
         if (l > (size_t)max_lds) throw std::runtime_error("batch product too large for the device's LDS");
-    // Phase-major tile lists; inside a phase, products in order.
+    // Groups of consecutive products; inside a group, phase-major tile lists
+    // with the products in order.
     const uint32_t P = (uint32_t)prods.size();
-    off[PH_RNS] = 0;
-    for (uint32_t i = 0; i < P; ++i)
-        for (uint32_t op = 0; op < 2; ++op)
-            for (uint32_t n = 0; n < prods[i].D.N; n += kBatchRnsTile) tasks.push_back({i, n, op});
-    off[PH_COLF] = (uint32_t)tasks.size();
-    for (uint32_t i = 0; i < P; ++i) {
-        const BatchProduct& B = prods[i];
-        if (!B.D.w1) continue;
-        for (uint32_t op = 0; op < 2; ++op)
+    const size_t cap = c.group_mb > 0 ? (size_t)c.group_mb << 20 : ~size_t(0);
+    off.clear();
+    single.clear();
+    single_cfg.clear();
+    for (uint32_t g0 = 0; g0 < P;) {
+        uint32_t g1 = g0;
+        size_t bytes = 0;
+        do {
+            bytes += 8 * (size_t)prods[g1].D.K * prods[g1].D.N;
+            ++g1;
+        } while (g1 < P && bytes + 8 * (size_t)prods[g1].D.K * prods[g1].D.N <= cap);
+        if (g1 == g0 + 1 && c.single_kernels) {
+            single.push_back((int)g0);
+            single_cfg.push_back(choose_launch(prods[g0].D, std::min<size_t>((size_t)max_lds, 64 * 1024)));
+            const LaunchCfg& L = single_cfg.back();
+            if (L.crt_lds > (size_t)max_lds || L.rns_lds > (size_t)max_lds)
+                throw std::runtime_error("batch product too large for the device's LDS");
+        } else {
+            single.push_back(-1);
+            single_cfg.push_back(LaunchCfg{});
+        }
+        off.push_back((uint32_t)tasks.size());  // RNS
+        for (uint32_t i = g0; i < g1; ++i)
+            for (uint32_t op = 0; op < 2; ++op)
+                for (uint32_t n = 0; n < prods[i].D.N; n += kBatchRnsTile) tasks.push_back({i, n, op});
+        off.push_back((uint32_t)tasks.size());  // forward columns
+        for (uint32_t i = g0; i < g1; ++i) {
+            const BatchProduct& B = prods[i];
+            if (!B.D.w1) continue;
+            for (uint32_t op = 0; op < 2; ++op)
+                for (int j = 0; j < B.D.K; ++j)
+                    for (uint32_t c0 = 0; c0 < (1u << B.D.w2); c0 += B.col_tc)
+                        tasks.push_back({i, c0, (uint32_t)j | (op << 16)});
+        }
+        off.push_back((uint32_t)tasks.size());  // fused rows
+        for (uint32_t i = g0; i < g1; ++i) {
+            const BatchProduct& B = prods[i];
             for (int j = 0; j < B.D.K; ++j)
-                for (uint32_t c0 = 0; c0 < (1u << B.D.w2); c0 += B.col_tc) tasks.push_back({i, c0, (uint32_t)j | (op << 16)});
+                for (uint32_t r0 = 0; r0 < (1u << B.D.w1); r0 += B.row_rpb) tasks.push_back({i, r0, (uint32_t)j});
+        }
+        off.push_back((uint32_t)tasks.size());  // inverse columns
+        for (uint32_t i = g0; i < g1; ++i) {
+            const BatchProduct& B = prods[i];
+            if (!B.D.w1) continue;
+            for (int j = 0; j < B.D.K; ++j)
+                for (uint32_t c0 = 0; c0 < (1u << B.D.w2); c0 += B.col_tc) tasks.push_back({i, c0, (uint32_t)j});
+        }
+        off.push_back((uint32_t)tasks.size());  // CRT
+        for (uint32_t i = g0; i < g1; ++i)
+            for (uint32_t n = 0; n < prods[i].D.N; n += kBatchCrtTile) tasks.push_back({i, n, 0});
+        g0 = g1;
     }
-    off[PH_ROW] = (uint32_t)tasks.size();
-    for (uint32_t i = 0; i < P; ++i) {
-        const BatchProduct& B = prods[i];
-        for (int j = 0; j < B.D.K; ++j)
-            for (uint32_t r0 = 0; r0 < (1u << B.D.w1); r0 += B.row_rpb) tasks.push_back({i, r0, (uint32_t)j});
-    }
-    off[PH_COLI] = (uint32_t)tasks.size();
-    for (uint32_t i = 0; i < P; ++i) {
-        const BatchProduct& B = prods[i];
-        if (!B.D.w1) continue;
-        for (int j = 0; j < B.D.K; ++j)
-            for (uint32_t c0 = 0; c0 < (1u << B.D.w2); c0 += B.col_tc) tasks.push_back({i, c0, (uint32_t)j});
-    }
-    off[PH_CRT] = (uint32_t)tasks.size();
-    for (uint32_t i = 0; i < P; ++i)
-        for (uint32_t n = 0; n < prods[i].D.N; n += kBatchCrtTile) tasks.push_back({i, n, 0});
-    off[PH_COUNT] = (uint32_t)tasks.size();
+    off.push_back((uint32_t)tasks.size());
 }
 
 void Runner::upload(hipStream_t s, bool pinned) {
     ensure(d_prods, prod_cap, prods.size(), "hipMalloc products");
     ensure(d_tasks, task_cap, tasks.size(), "hipMalloc tiles");
-    if (!d_ctr) hip_check(hipMalloc(&d_ctr, PH_COUNT * sizeof(uint32_t)), "hipMalloc counters");
+    ensure(d_ctr, ctr_cap, std::max<size_t>(1, off.size() - 1), "hipMalloc counters");
     const void* hp = prods.data();
     const void* ht = tasks.data();
     if (pinned) {
@@ -224,6 +249,7 @@ void Runner::upload(hipStream_t s, bool pinned) {
 }
 
 void Runner::reserve(size_t nprods, size_t ntasks) {
+    ensure(d_ctr, ctr_cap, PH_COUNT * nprods, "hipMalloc counters");
     ensure(d_prods, prod_cap, nprods, "hipMalloc products");
     ensure(d_tasks, task_cap, ntasks, "hipMalloc tiles");
     ensure_pinned(h_prods, h_prod_cap, nprods);
@@ -232,19 +258,38 @@ void Runner::reserve(size_t nprods, size_t ntasks) {
     std::memset(static_cast<void*>(h_tasks), 0, h_task_cap * sizeof(BatchTask));
 }
 
-hipError_t Runner::launch(hipStream_t s) const {
+hipError_t Runner::launch_reset(hipStream_t s) const {
+    if (cfg.blocks_per_cu <= 0) return hipSuccess;
+    return launch_batch_reset(d_ctr, (uint32_t)(off.size() - 1), s);
+}
+
+hipError_t Runner::launch_group(size_t g, hipStream_t s) const {
     const bool persistent = cfg.blocks_per_cu > 0;
-    if (persistent) {
-        const hipError_t e = launch_batch_reset(d_ctr, PH_COUNT, s);
-        if (e != hipSuccess) return e;
-    }
     const int grid = std::max(1, cus * cfg.blocks_per_cu);
+    if (single[g] >= 0) {
+        // the plain multiply's sequence
+        const BatchProduct& B = prods[single[g]];
+        const LaunchCfg& C = single_cfg[g];
+        hipError_t e;
+        if ((e = launch_rns(B.D, C, B.a, B.b, B.R0, B.R1, 2, s)) != hipSuccess) return e;
+        if ((e = launch_col(B.D, C, B.R0, B.R1, 2, false, s)) != hipSuccess) return e;
+        if ((e = launch_row(B.D, C, B.R1, B.R0, ROW_FUSED, s)) != hipSuccess) return e;
+        if ((e = launch_col(B.D, C, B.R1, nullptr, 1, true, s)) != hipSuccess) return e;
+        return launch_crt(B.D, C, B.R1, B.c, B.D.G, s);
+    }
     for (int ph = 0; ph < PH_COUNT; ++ph) {
-        const hipError_t e = launch_batch_phase(ph, d_prods, d_tasks + off[ph], off[ph + 1] - off[ph],
-                                                persistent ? d_ctr + ph : nullptr, grid, lds[ph], s);
+        const size_t k = g * PH_COUNT + ph;
+        const hipError_t e = launch_batch_phase(ph, d_prods, d_tasks + off[k], off[k + 1] - off[k],
+                                                persistent ? d_ctr + k : nullptr, grid, lds[ph], s);
         if (e != hipSuccess) return e;
     }
     return hipSuccess;
+}
+
+hipError_t Runner::launch(hipStream_t s) const {
+    hipError_t e = launch_reset(s);
+    for (size_t g = 0; e == hipSuccess && g < groups(); ++g) e = launch_group(g, s);
+    return e;
 }
 
 Runner::~Runner() {
@@ -263,6 +308,8 @@ struct Batch {
     Runner run;
     uint32_t* d_R = nullptr;
     hipStream_t stream = nullptr;
+    hipStream_t stream2 = nullptr;  // second branch, used only inside the capture
+    hipEvent_t fork = nullptr, join = nullptr;
     hipGraph_t graph = nullptr;
     hipGraphExec_t exec = nullptr;
 
@@ -270,6 +317,9 @@ struct Batch {
         if (stream) (void)hipStreamSynchronize(stream);
         drop_graph();
         if (d_R) (void)hipFree(d_R);
+        if (fork) (void)hipEventDestroy(fork);
+        if (join) (void)hipEventDestroy(join);
+        if (stream2) (void)hipStreamDestroy(stream2);
         if (stream) (void)hipStreamDestroy(stream);
     }
     void drop_graph() {
@@ -306,7 +356,18 @@ struct Batch {
     void enqueue(hipStream_t st) {
         if (!exec) {
             hip_check(hipStreamBeginCapture(stream, hipStreamCaptureModeThreadLocal), "capture");
-            const hipError_t e = run.launch(stream);
+            hipError_t e = run.launch_reset(stream);
+            if (run.cfg.branches > 1) {
+                // groups alternate over two chains that fork and join inside the graph
+                if (e == hipSuccess) e = hipEventRecord(fork, stream);
+                if (e == hipSuccess) e = hipStreamWaitEvent(stream2, fork, 0);
+                for (size_t g = 0; e == hipSuccess && g < run.groups(); ++g)
+                    e = run.launch_group(g, g % 2 ? stream2 : stream);
+                if (e == hipSuccess) e = hipEventRecord(join, stream2);
+                if (e == hipSuccess) e = hipStreamWaitEvent(stream, join, 0);
+            } else {
+                for (size_t g = 0; e == hipSuccess && g < run.groups(); ++g) e = run.launch_group(g, stream);
+            }
             hipGraph_t g = nullptr;
             const hipError_t e2 = hipStreamEndCapture(stream, &g);
             hip_check(e, "batch launch");
@@ -338,6 +399,9 @@ void batch_build(Batch& B, uint32_t count, const uint32_t* N, const uint32_t* W,
     B.cat.build(count, N, W, q_concat);
     B.run.init();
     hip_check(hipStreamCreateWithFlags(&B.stream, hipStreamNonBlocking), "stream");
+    hip_check(hipStreamCreateWithFlags(&B.stream2, hipStreamNonBlocking), "stream");
+    hip_check(hipEventCreateWithFlags(&B.fork, hipEventDisableTiming), "event");
+    hip_check(hipEventCreateWithFlags(&B.join, hipEventDisableTiming), "event");
     size_t rwords = 0;
     for (size_t i = 0; i < count; ++i) rwords += B.cat.residue_words(i);
     hip_check(hipMalloc(&B.d_R, rwords * 4), "hipMalloc residues");
@@ -444,6 +508,23 @@ int pm_batch_time(pm_batch* b, const uint32_t* const* d_a, const uint32_t* const
     });
 }
 
+int pm_batch_configure(pm_batch* b, int col_tile, int ceil_split, int blocks_per_cu, int group_mb, int branches) {
+    return guarded([&] {
+        if (!b) throw std::runtime_error("null argument");
+        if (col_tile != 8 && col_tile != 16 && col_tile != 32) throw std::runtime_error("col_tile must be 8, 16 or 32");
+        if (ceil_split < 0 || ceil_split > 1 || blocks_per_cu < 0 || blocks_per_cu > 8 || group_mb < 0 ||
+            branches < 1 || branches > 2)
+            throw std::runtime_error("batch configuration out of range");
+        pm::BatchConfig c;
+        c.col_tc = col_tile;
+        c.ceil_split = ceil_split;
+        c.blocks_per_cu = blocks_per_cu;
+        c.group_mb = group_mb;
+        c.branches = branches;
+        b->b->configure(c);
+    });
+}
+
 int pm_batch_autotune(pm_batch* b, const uint32_t* const* d_a, const uint32_t* const* d_b, uint32_t* const* d_c,
                       int iters, int* best) {
     return guarded([&] {
@@ -453,11 +534,15 @@ int pm_batch_autotune(pm_batch* b, const uint32_t* const* d_a, const uint32_t* c
         float bestms = 1e30f;
         for (int tc : {8, 16, 32})
             for (int split = 0; split < 2; ++split)
-                for (int bpc : {0, 2, 4}) {
+                for (int bpc : {0, 2})
+                    for (int gmb : {0, 48, 96, 192})
+                        for (int br : {1, 2}) {
                     pm::BatchConfig c;
                     c.col_tc = tc;
                     c.ceil_split = split;
                     c.blocks_per_cu = bpc;
+                    c.group_mb = gmb;
+                    c.branches = br;
                     try {
                         B.configure(c);
                     } catch (const std::runtime_error&) {
@@ -475,6 +560,8 @@ int pm_batch_autotune(pm_batch* b, const uint32_t* const* d_a, const uint32_t* c
             best[1] = bestc.ceil_split;
             best[2] = bestc.blocks_per_cu;
             best[3] = (int)(bestms * 1000.0f);  // microseconds
+            best[4] = bestc.group_mb;
+            best[5] = bestc.branches;
         }
     });
 }

@@ -460,7 +460,8 @@ __device__ __forceinline__ void crt_tile(const DevPlan& D, const u32* __restrict
     const int L = D.L, K = D.K, nslot = L + 3, ZL = L + 2;
     const u32 N = D.N;
     unsigned char* Y = lds8;                                           // [TN][ys]
-    long long* Pp = reinterpret_cast<long long*>(lds8 + (size_t)TN * ys);  // [TN][nslot]
+    // slot-major, so lanes (consecutive coefficients) touch consecutive words
+    long long* Pp = reinterpret_cast<long long*>(lds8 + (size_t)TN * ys);  // [nslot][TN]
     const int cnt = (int)min((u32)TN, count - n0);
 
     // y bytes: thread (c, r) covers primes j = r, r + JS, ... for coefficient c,
@@ -509,7 +510,7 @@ __device__ __forceinline__ void crt_tile(const DevPlan& D, const u32* __restrict
             const int slot = 8 * rt + 2 * g + half;
             if (slot >= nslot) continue;
 #pragma unroll
-            for (int ct = 0; ct < NCT; ++ct) Pp[(size_t)(32 * ct + col) * nslot + slot] = slot_value(acc[ct], g);
+            for (int ct = 0; ct < NCT; ++ct) Pp[(size_t)slot * TN + 32 * ct + col] = slot_value(acc[ct], g);
         }
     }
     __syncthreads();
@@ -518,9 +519,9 @@ __device__ __forceinline__ void crt_tile(const DevPlan& D, const u32* __restrict
     const int W = D.W, wl = W >> 5, wb = W & 31;
     const u32* C0 = D.C0;
     for (int c = tid; c < cnt; c += nthreads) {
-        const long long* P = Pp + (size_t)c * nslot;
+        const long long* P = Pp + c;  // slot i at P[i * TN]
         u32* z = Z + (size_t)c * ZL;
-        const u64 F = (u64)P[L + 1] + ((u64)P[L + 2] << 32) + (u64)D.F0;
+        const u64 F = (u64)P[(L + 1) * TN] + ((u64)P[(L + 2) * TN] << 32) + (u64)D.F0;
         // v <= K for every valid input; the clamp only keeps a corrupted sum
         // from indexing past T.
         const u32 v = min((u32)((F + (1ull << 47)) >> 48), (u32)K);
@@ -528,12 +529,12 @@ __device__ __forceinline__ void crt_tile(const DevPlan& D, const u32* __restrict
         long long carry = 0;
 #pragma unroll 8
         for (int i = 0; i < L; ++i) {
-            const long long s = carry + P[i] + (long long)C0[i] + (long long)Tv[i];
+            const long long s = carry + P[i * TN] + (long long)C0[i] + (long long)Tv[i];
             z[i] = (u32)s;
             carry = s >> 32;  // arithmetic: the running sum may be negative
         }
         {
-            long long s = carry + P[L] + (long long)C0[L];
+            long long s = carry + P[L * TN] + (long long)C0[L];
             z[L] = (u32)s;
             s = (s >> 32) + (long long)C0[L + 1];
             z[L + 1] = (u32)s;
@@ -634,7 +635,9 @@ __device__ __forceinline__ void row_tile_nb(int nb, const DevPlan& D, u32* RB, c
 
 template <int PH>
 __device__ __forceinline__ void batch_task(const BatchProduct& P, u32 x, u32 y, u32* lds, int tid) {
-    const DevPlan& D = P.D;
+    // A copy, so the plan's fields are loaded once per tile (uniform, into
+    // scalar registers) rather than re-read through the product pointer.
+    const DevPlan D = P.D;
     if constexpr (PH == PH_RNS) {
         rns_tile<kBatchRnsTile / 32>(D, y ? P.b : P.a, y ? P.R1 : P.R0, x, D.N, 1, lds, tid, kBatchThreads[PH]);
     } else if constexpr (PH == PH_COLF) {

@@ -24,7 +24,7 @@ C_SYMBOLS = (
     "pm_dev_alloc", "pm_dev_free", "pm_copy_to_device", "pm_copy_to_host", "pm_sync", "pm_time_device",
     "pm_stage_rns", "pm_stage_ntt_fwd", "pm_stage_ntt_inv", "pm_stage_crt", "pm_probe_transfers",
     "pm_batch_create", "pm_batch_free", "pm_batch_last_error", "pm_batch_product_info", "pm_batch_run",
-    "pm_batch_sync", "pm_batch_time", "pm_batch_autotune",
+    "pm_batch_sync", "pm_batch_time", "pm_batch_autotune", "pm_batch_configure",
     "pm_server_create", "pm_server_free", "pm_server_last_error", "pm_server_submit", "pm_server_wait",
     "pm_server_stats_get", "pm_server_drive",
 )
@@ -125,6 +125,7 @@ def load() -> ctypes.CDLL:
         "pm_batch_sync": ([_vp], ctypes.c_int),
         "pm_batch_time": ([_vp, _vpp, _vpp, _vpp, ctypes.c_int, ctypes.POINTER(ctypes.c_float)], ctypes.c_int),
         "pm_batch_autotune": ([_vp, _vpp, _vpp, _vpp, ctypes.c_int, ctypes.POINTER(ctypes.c_int)], ctypes.c_int),
+        "pm_batch_configure": ([_vp] + [ctypes.c_int] * 5, ctypes.c_int),
         "pm_server_create": ([ctypes.c_uint32, _u32p, _u32p, _u32p, ctypes.POINTER(ServerConfig),
                               ctypes.POINTER(_vp)], ctypes.c_int),
         "pm_server_free": ([_vp], None),
@@ -347,10 +348,14 @@ class Batch:
                                           ctypes.byref(ms)))
         return ms.value
 
+    def configure(self, col_tile=32, ceil_split=0, blocks_per_cu=0, group_mb=192, branches=1):
+        _check_batch(load().pm_batch_configure(self._h, col_tile, ceil_split, blocks_per_cu, group_mb, branches))
+
     def autotune(self, da, db, dc, iters: int = 3) -> dict:
-        best = (ctypes.c_int * 4)()
+        best = (ctypes.c_int * 6)()
         _check_batch(load().pm_batch_autotune(self._h, self._ptrs(da), self._ptrs(db), self._ptrs(dc), iters, best))
-        return {"col_tile": best[0], "ceil_split": best[1], "blocks_per_cu": best[2], "us_per_launch": best[3]}
+        return {"col_tile": best[0], "ceil_split": best[1], "blocks_per_cu": best[2], "us_per_launch": best[3],
+                "group_mb": best[4], "branches": best[5]}
 
 
 def _check_server(rc: int):

@@ -84,3 +84,24 @@ def test_batch_closed_form_full_size(O, pm):
     want = O.all_qm1(N, q)
     for i in range(8):
         assert O.unpack(dc[i].download((N, L))) == want, i
+
+
+@pytest.mark.parametrize("cfg", [
+    dict(col_tile=8, ceil_split=1, blocks_per_cu=2, group_mb=0, branches=1),
+    dict(col_tile=16, ceil_split=0, blocks_per_cu=0, group_mb=1, branches=2),
+    dict(col_tile=32, ceil_split=1, blocks_per_cu=2, group_mb=4, branches=2),
+    dict(col_tile=32, ceil_split=0, blocks_per_cu=0, group_mb=0, branches=2),
+])
+def test_batch_configurations(O, pm, cfg):
+    """Every hand-out mode, grouping (including groups of one product, which
+    run the single-product kernels) and graph branching gives the same bits."""
+    points = [(4096, 256), (2048, 868), (8192, 438), (1024, 1747), (32768, 300), (16, 64), (2048, 2048)]
+    print("config", cfg, flush=True)
+    B, prods, ops, da, db, dc = _batch(O, pm, points, 500, "mixed")
+    B.configure(**cfg)
+    for _ in range(2):
+        for d in dc:
+            d.upload(np.zeros(d.nbytes // 4, np.uint32))
+        B.run(da, db, dc)
+        B.sync()
+        _check(O, prods, ops, dc)
