@@ -140,15 +140,42 @@ def serving(pmlib, O, rates, seconds: float) -> dict:
     out = {"mix": {"%d/%d" % p: f for p, f in zip(SERVE_POINTS, SERVE_MIX)}, "seconds_per_rate": seconds,
            "bytes_per_request_mean": round(sum(f * 12 * n * O.limbs_of(w) for (n, w), f in zip(SERVE_POINTS, SERVE_MIX))),
            "rates": []}
+    def row_of(rep):
+        p50, p99 = rep.pop("point_p50_ms"), rep.pop("point_p99_ms")
+        row = {k: (round(v, 3) if isinstance(v, float) else v) for k, v in rep.items()}
+        row["per_point"] = {"%d/%d" % pt: {"p50_ms": round(p50[i], 3), "p99_ms": round(p99[i], 3)}
+                            for i, pt in enumerate(SERVE_POINTS)}
+        return row
+
     with pmlib.Server(cat) as S:
         S.drive(200.0, 0.5, SERVE_MIX, seed=SEED)  # warm-up
         for r in rates:
-            rep = S.drive(float(r), seconds, SERVE_MIX, seed=SEED + int(r))
-            p50, p99 = rep.pop("point_p50_ms"), rep.pop("point_p99_ms")
-            row = {k: (round(v, 3) if isinstance(v, float) else v) for k, v in rep.items()}
-            row["per_point"] = {"%d/%d" % pt: {"p50_ms": round(p50[i], 3), "p99_ms": round(p99[i], 3)}
-                                for i, pt in enumerate(SERVE_POINTS)}
-            out["rates"].append(row)
+            out["rates"].append(row_of(S.drive(float(r), seconds, SERVE_MIX, seed=SEED + int(r))))
+        # capacity: closed loop with 64 requests in flight; the achieved rate
+        # is the server's throughput on this mix
+        cap = row_of(S.saturate(64, min(seconds, 3.0), SERVE_MIX, seed=SEED + 1))
+        out["capacity"] = {k: cap[k] for k in ("requests", "achieved_rate", "mean_batch", "p50_ms", "p99_ms",
+                                               "failed")}
+        out["capacity"]["in_flight"] = 64
+    # the plain per-call path on the same mix: pm_run (the fherma_run
+    # replacement) from pageable host buffers, one request at a time
+    per_point = {}
+    for (n, w, q) in cat:
+        L = O.limbs_of(w)
+        a, b = O.operands("uniform", n, w, q, SEED + 7)
+        c = np.empty((n, L), np.uint32)
+        with pmlib.Multiplier(n, w, O.q_limbs(q, L)) as m:
+            m.run(a, b, c)
+            ts = []
+            for _ in range(10):
+                t = time.perf_counter()
+                m.run(a, b, c)
+                ts.append(time.perf_counter() - t)
+        per_point["%d/%d" % (n, w)] = float(np.median(ts))
+    mean_s = sum(f * per_point["%d/%d" % p] for p, f in zip(SERVE_POINTS, SERVE_MIX))
+    out["plain_per_call"] = {"median_ms": {k: round(1e3 * v, 3) for k, v in per_point.items()},
+                             "mix_req_per_s": round(1.0 / mean_s, 1)}
+    out["capacity_vs_plain"] = round(out["capacity"]["achieved_rate"] / (1.0 / mean_s), 3)
     return out
 
 

@@ -192,6 +192,11 @@ int pm_server_stats_get(pm_server* s, int reset, pm_server_stats* out);
  * generated from `seed`; latency counts from each scheduled arrival. */
 int pm_server_drive(pm_server* s, double rate, double seconds, const double* weights, uint64_t seed,
                     pm_serve_report* out);
+/* Closed-loop load: `depth` requests kept in flight for `seconds`, a new one
+ * submitted as each completes; achieved_rate is the server's capacity on
+ * the mix (offered_rate is reported as 0). */
+int pm_server_saturate(pm_server* s, uint32_t depth, double seconds, const double* weights, uint64_t seed,
+                       pm_serve_report* out);
 
 #ifdef __cplusplus
 }

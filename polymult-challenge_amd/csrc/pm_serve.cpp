@@ -7,7 +7,7 @@
 //   stage (host team: pageable -> pinned) -> one H2D per request ->
 //   five phase launches over every request's tiles -> one D2H per request ->
 //   scatter (host team: pinned -> the caller's buffer),
-// on three streams (in, compute, out) with up to `max_inflight` batches at
+// on four streams (in, two for compute, out) with up to `max_inflight` batches at
 // once, so one batch's transfers overlap another's compute. Latency is
 // measured from a request's arrival to the end of its scatter.
 #include <hip/hip_runtime.h>
@@ -135,7 +135,11 @@ struct Server {
     Arena dev_arena, pin_arena;
     size_t max_batch = 64;
     size_t max_batch_bytes = 0;
-    hipStream_t s_in = nullptr, s_comp = nullptr, s_out = nullptr;
+    // Copies in and out on one stream each; consecutive batches compute on
+    // alternate streams so their kernels can overlap (four streams: the
+    // hardware queues a process gets by default).
+    hipStream_t s_in = nullptr, s_out = nullptr, s_comp[2] = {nullptr, nullptr};
+    uint64_t batch_seq = 0;  // scheduler thread only
     std::unique_ptr<Team> stage_team, scatter_team;
 
     struct Slot {
@@ -231,7 +235,8 @@ void Server::create(uint32_t npoints, const uint32_t* N, const uint32_t* W, cons
     dev_arena.reset(dev_budget);
     pin_arena.reset(pin_budget);
     hip_check(hipStreamCreateWithFlags(&s_in, hipStreamNonBlocking), "stream");
-    hip_check(hipStreamCreateWithFlags(&s_comp, hipStreamNonBlocking), "stream");
+    hip_check(hipStreamCreateWithFlags(&s_comp[0], hipStreamNonBlocking), "stream");
+    hip_check(hipStreamCreateWithFlags(&s_comp[1], hipStreamNonBlocking), "stream");
     hip_check(hipStreamCreateWithFlags(&s_out, hipStreamNonBlocking), "stream");
     for (int i = 0; i < ninflight; ++i) {
         auto S = std::make_unique<Slot>();
@@ -278,13 +283,13 @@ Server::~Server() {
     if (sched.joinable()) sched.join();
     if (completer.joinable()) completer.join();
     if (!trace_path.empty()) write_trace();
-    for (hipStream_t s : {s_in, s_comp, s_out})
+    for (hipStream_t s : {s_in, s_comp[0], s_comp[1], s_out})
         if (s) (void)hipStreamSynchronize(s);
     for (auto& S : slots)
         for (hipEvent_t e : {S->ev_in, S->ev_comp, S->ev_out})
             if (e) (void)hipEventDestroy(e);
     slots.clear();
-    for (hipStream_t s : {s_in, s_comp, s_out})
+    for (hipStream_t s : {s_in, s_comp[0], s_comp[1], s_out})
         if (s) (void)hipStreamDestroy(s);
     if (d_arena) (void)hipFree(d_arena);
     if (h_arena) (void)hipHostFree(h_arena);
@@ -447,10 +452,11 @@ void Server::launch(Slot& S) {
         P.R1 = P.R0 + res_bytes[r->point] / 4;
     }
     R.plan(cfg);
-    hip_check(hipStreamWaitEvent(s_comp, S.ev_in, 0), "wait");
-    R.upload(s_comp, true);
-    hip_check(R.launch(s_comp), "batch launch");
-    hip_check(hipEventRecord(S.ev_comp, s_comp), "record");
+    hipStream_t sc = s_comp[batch_seq++ % 2];
+    hip_check(hipStreamWaitEvent(sc, S.ev_in, 0), "wait");
+    R.upload(sc, true);
+    hip_check(R.launch(sc), "batch launch");
+    hip_check(hipEventRecord(S.ev_comp, sc), "record");
 
     hip_check(hipStreamWaitEvent(s_out, S.ev_comp, 0), "wait");
     for (Request* r : S.reqs)
@@ -579,7 +585,7 @@ void Server::schedule_loop() {
             launch(*S);
         } catch (const std::exception& e) {
             // drain what was already issued before reusing anything
-            for (hipStream_t s : {s_in, s_comp, s_out}) (void)hipStreamSynchronize(s);
+            for (hipStream_t s : {s_in, s_comp[0], s_comp[1], s_out}) (void)hipStreamSynchronize(s);
             fail_batch(*S, e.what());
             std::lock_guard<std::mutex> lk(m);
             idle.push_back(S);
@@ -635,8 +641,13 @@ std::vector<uint32_t> random_operand(std::mt19937_64& g, uint32_t N, uint32_t W)
     return v;
 }
 
-void drive(Server& S, double rate, double seconds, const double* weights, uint64_t seed, pm_serve_report* out) {
-    if (!(rate > 0) || !(seconds > 0)) throw std::runtime_error("rate and seconds must be positive");
+// Open loop (depth 0): Poisson arrivals at `rate`, latency from each
+// scheduled arrival. Closed loop (depth > 0): `depth` requests kept in
+// flight, a new one submitted as soon as one completes; the achieved rate is
+// then the server's capacity on the mix.
+void drive(Server& S, double rate, uint32_t depth, double seconds, const double* weights, uint64_t seed,
+           pm_serve_report* out) {
+    if ((depth == 0 && !(rate > 0)) || !(seconds > 0)) throw std::runtime_error("rate and seconds must be positive");
     const size_t np = S.cat.plans.size();
     std::vector<double> w(weights, weights + np);
     double wsum = 0;
@@ -657,10 +668,10 @@ void drive(Server& S, double rate, double seconds, const double* weights, uint64
     std::vector<std::vector<std::unique_ptr<std::vector<uint32_t>>>> pool(np);
     for (size_t i = 0; i < np; ++i)
         if (w[i] > 0)
-            for (int k = (int)std::ceil(64.0 * w[i] / wsum) + 2; k > 0; --k)
+            for (int k = (int)std::ceil((depth ? depth : 64) * w[i] / wsum) + 2; k > 0; --k)
                 pool[i].push_back(std::make_unique<std::vector<uint32_t>>(S.cat.io_words(i), 1u));
     std::discrete_distribution<int> pick(w.begin(), w.end());
-    std::exponential_distribution<double> gap(rate);
+    std::exponential_distribution<double> gap(depth ? 1.0 : rate);
 
     struct Out {
         uint64_t id;
@@ -705,17 +716,7 @@ void drive(Server& S, double rate, double seconds, const double* weights, uint64
     rusage ru0{}, rt0{};
     getrusage(RUSAGE_SELF, &ru0);
     getrusage(RUSAGE_THREAD, &rt0);
-    const double t0 = now_s() + 0.02;
-    double t = 0;
-    for (;;) {
-        t += gap(g);
-        if (t >= seconds) break;
-        const double target = t0 + t;
-        for (double d; (d = target - now_s()) > 0;) {
-            if (d > 300e-6) std::this_thread::sleep_for(std::chrono::duration<double>(d - 200e-6));
-            else relax();
-        }
-        max_late = std::max(max_late, now_s() - target);
+    auto submit_one = [&](double t_arrive) {
         const uint32_t p = (uint32_t)pick(g);
         if (pool[p].empty())
             pool[p].push_back(std::make_unique<std::vector<uint32_t>>(S.cat.io_words(p)));
@@ -723,10 +724,51 @@ void drive(Server& S, double rate, double seconds, const double* weights, uint64
         pool[p].pop_back();
         const auto& op = ops[p];
         const size_t k = (size_t)(nreq % 2) * 2;
-        o.id = S.submit(p, op[k].data(), op[k + 1].data(), o.buf->data(), target);
+        o.id = S.submit(p, op[k].data(), op[k + 1].data(), o.buf->data(), t_arrive);
         outstanding.push_back(std::move(o));
         ++nreq;
-        if (outstanding.size() > 8) reap(false);
+    };
+    const double t0 = now_s() + 0.02;
+    if (depth > 0) {
+        while (now_s() < t0) relax();
+        while (now_s() - t0 < seconds) {
+            while (outstanding.size() < depth) submit_one(now_s());
+            // wait for the oldest, then take whatever else has finished
+            double ms = 0, qms = 0;
+            Out& f = outstanding.front();
+            int rc;
+            try {
+                rc = S.wait(f.id, 60.0, &ms, &qms);
+            } catch (const std::exception&) {
+                rc = -1;
+            }
+            if (rc == 1) throw std::runtime_error("request did not complete within 60 s");
+            if (rc == 0) {
+                lat.push_back(ms);
+                qlat.push_back(qms);
+                slat.push_back(ms - qms);
+                plat[f.point].push_back(ms);
+            } else {
+                ++failed;
+            }
+            pool[f.point].push_back(std::move(f.buf));
+            outstanding.pop_front();
+            reap(false);
+        }
+    } else {
+        double t = 0;
+        for (;;) {
+            t += gap(g);
+            if (t >= seconds) break;
+            const double target = t0 + t;
+            for (double d; (d = target - now_s()) > 0;) {
+                if (d > 300e-6) std::this_thread::sleep_for(std::chrono::duration<double>(d - 200e-6));
+                else relax();
+            }
+            max_late = std::max(max_late, now_s() - target);
+            submit_one(target);
+            if (outstanding.size() > 8) reap(false);
+        }
     }
     reap(true);
     const double t_end = now_s();
@@ -737,7 +779,7 @@ void drive(Server& S, double rate, double seconds, const double* weights, uint64
     pm_server_stats st;
     S.stats(&st, true);
     *out = pm_serve_report{};
-    out->offered_rate = rate;
+    out->offered_rate = depth ? 0.0 : rate;
     out->seconds = seconds;
     out->requests = nreq;
     out->completed = lat.size();
@@ -852,7 +894,17 @@ int pm_server_drive(pm_server* s, double rate, double seconds, const double* wei
                     pm_serve_report* out) {
     return guarded([&] {
         if (!s || !weights || !out) throw std::runtime_error("null argument");
-        pm::drive(*s->s, rate, seconds, weights, seed, out);
+        pm::drive(*s->s, rate, 0, seconds, weights, seed, out);
+        return 0;
+    });
+}
+
+int pm_server_saturate(pm_server* s, uint32_t depth, double seconds, const double* weights, uint64_t seed,
+                       pm_serve_report* out) {
+    return guarded([&] {
+        if (!s || !weights || !out) throw std::runtime_error("null argument");
+        if (depth == 0 || depth > 4096) throw std::runtime_error("depth must be in 1..4096");
+        pm::drive(*s->s, 0.0, depth, seconds, weights, seed, out);
         return 0;
     });
 }

@@ -26,7 +26,7 @@ C_SYMBOLS = (
     "pm_batch_create", "pm_batch_free", "pm_batch_last_error", "pm_batch_product_info", "pm_batch_run",
     "pm_batch_sync", "pm_batch_time", "pm_batch_autotune", "pm_batch_configure",
     "pm_server_create", "pm_server_free", "pm_server_last_error", "pm_server_submit", "pm_server_wait",
-    "pm_server_stats_get", "pm_server_drive",
+    "pm_server_stats_get", "pm_server_drive", "pm_server_saturate",
 )
 # The C++ entry points of include/fherma.h, mangled as the reference's harness binds them.
 CXX_SYMBOLS = (
@@ -136,6 +136,8 @@ def load() -> ctypes.CDLL:
         "pm_server_stats_get": ([_vp, ctypes.c_int, ctypes.POINTER(ServerStats)], ctypes.c_int),
         "pm_server_drive": ([_vp, ctypes.c_double, ctypes.c_double, ctypes.POINTER(ctypes.c_double), ctypes.c_uint64,
                              ctypes.POINTER(ServeReport)], ctypes.c_int),
+        "pm_server_saturate": ([_vp, ctypes.c_uint32, ctypes.c_double, ctypes.POINTER(ctypes.c_double),
+                                ctypes.c_uint64, ctypes.POINTER(ServeReport)], ctypes.c_int),
     }
     for name, (args, res) in sig.items():
         f = getattr(lib, name)
@@ -441,4 +443,11 @@ class Server:
         w = (ctypes.c_double * len(self.points))(*[float(x) for x in weights])
         rep = ServeReport()
         _check_server(load().pm_server_drive(self._h, rate, seconds, w, seed, ctypes.byref(rep)))
+        return _as_dict(rep)
+
+    def saturate(self, depth: int, seconds: float, weights, seed: int = 1) -> dict:
+        """Closed-loop load with `depth` requests in flight; see pm_server_saturate."""
+        w = (ctypes.c_double * len(self.points))(*[float(x) for x in weights])
+        rep = ServeReport()
+        _check_server(load().pm_server_saturate(self._h, depth, seconds, w, seed, ctypes.byref(rep)))
         return _as_dict(rep)

@@ -96,3 +96,13 @@ def test_serve_poisson_drive_short(O, pm):
         a, b = O.operands("mixed", N, W, q, 9)
         c, _ = S.wait(S.submit(0, a, b))
         assert np.array_equal(c, O.OraclePlan(N, W, q).multiply(a, b))
+
+
+def test_serve_saturate_closed_loop(O, pm):
+    """Closed-loop load keeps `depth` requests in flight and completes them all."""
+    cat = _catalog(O, MIX_POINTS[:3])
+    with pm.Server(cat) as S:
+        rep = S.saturate(16, 0.5, [0.70, 0.25, 0.05], seed=3)
+        print("saturate", {k: rep[k] for k in ("requests", "completed", "achieved_rate", "mean_batch")}, flush=True)
+        assert rep["completed"] == rep["requests"] > 16 and rep["failed"] == 0
+        assert rep["mean_batch"] > 1.0
