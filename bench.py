@@ -9,6 +9,7 @@ timed steps. The PCIe-inclusive rate of the host-buffer call (pm_run, the
 fherma_run replacement) is reported beside it, never as `value`.
 
     python bench.py [--gpus 1] [--steps 200] [--warmup 20] [--no-cpu] [--serve-seconds 10] [--no-serve]
+                    [--legs plain,e2e,batch,serve,cpu] [--copy-probe BYTES]
 
 Prints one JSON line.
 """
@@ -85,8 +86,8 @@ def cpu_baseline(q: int, sample_cases: int) -> dict | None:
             "per_call_s": secs, "output_matches_oracle": ok}
 
 
-def batched(pmlib, O, count: int, points, seed: int, check: int, iters: int) -> dict:
-    """One persistent hipGraph launch over `points`; mults/s = products / launch time."""
+def batched(pmlib, O, count: int, points, seed: int, check: int, iters: int, tune: bool = True) -> dict:
+    """One hipGraph launch over `points`; mults/s = products / launch time."""
     prods, ops = [], []
     for i, (n, w) in enumerate(points):
         q = O.spec_q(n, w)
@@ -96,7 +97,7 @@ def batched(pmlib, O, count: int, points, seed: int, check: int, iters: int) -> 
     da = [pmlib.DeviceBuffer.from_array(a) for a, _ in ops]
     db = [pmlib.DeviceBuffer.from_array(b) for _, b in ops]
     dc = [pmlib.DeviceBuffer(n * O.limbs_of(w) * 4) for n, w in points]
-    tuned = B.autotune(da, db, dc, 3)
+    tuned = B.autotune(da, db, dc, 3) if tune else "default configuration"
     B.run(da, db, dc)
     B.sync()
     exact = True
@@ -189,14 +190,28 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--serve-seconds", type=float, default=10.0)
     ap.add_argument("--no-serve", action="store_true")
+    ap.add_argument("--legs", default="plain,e2e,batch,serve,cpu",
+                    help="comma list of parts to run (plain is always run: it gives value)")
+    ap.add_argument("--no-autotune", action="store_true", help="batched legs on the default configuration")
+    ap.add_argument("--copy-probe", type=int, default=0, metavar="BYTES",
+                    help="only run the plain device copy kernel over BYTES (the counter calibration) and exit")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: this build runs on a single MI355X (the path would shard by RNS "
               "prime across GPUs, see the design notes); --gpus %d is not supported" % args.gpus, file=sys.stderr)
         sys.exit(2)
+    legs = set(args.legs.split(","))
+    if args.no_serve:
+        legs.discard("serve")
+    if args.no_cpu:
+        legs.discard("cpu")
 
     import pmlib
     from oracle import oracle as O
+
+    if args.copy_probe:
+        print(json.dumps({"copy_bytes": args.copy_probe, "GBps": round(pmlib.probe_copy(args.copy_probe, 10), 1)}))
+        return
 
     q = O.spec_q(N, W)
     L = O.limbs_of(W)
@@ -241,30 +256,35 @@ def main():
                       "Gmodmul_per_s": round(mm[k] / (kms[k] * 1e-3) / 1e9, 1)} for k in KERNELS}
 
     # end to end from pageable host buffers (the fherma_run replacement)
-    c_host = np.empty((N, L), np.uint32)
-    m.run(a, b, c_host)
-    e2e = []
-    for _ in range(args.e2e_calls):
-        t = time.perf_counter()
-        m.run(a, b, c_host)
-        e2e.append(time.perf_counter() - t)
-    e2e_med = float(np.median(e2e))
     pcie_bytes = 12 * N * L
-    end_to_end = {"mults_per_s": round(1.0 / e2e_med, 2), "median_ms": round(1e3 * e2e_med, 3),
-                  "pct_pcie_peak": round(100 * pcie_bytes / e2e_med / (PCIE_PEAK_GBS * 1e9), 1),
-                  "bit_exact": bool(np.array_equal(c_host, c_dev))}
+    end_to_end = None
+    if "e2e" in legs:
+        c_host = np.empty((N, L), np.uint32)
+        m.run(a, b, c_host)
+        e2e = []
+        for _ in range(args.e2e_calls):
+            t = time.perf_counter()
+            m.run(a, b, c_host)
+            e2e.append(time.perf_counter() - t)
+        e2e_med = float(np.median(e2e))
+        end_to_end = {"mults_per_s": round(1.0 / e2e_med, 2), "median_ms": round(1e3 * e2e_med, 3),
+                      "pct_pcie_peak": round(100 * pcie_bytes / e2e_med / (PCIE_PEAK_GBS * 1e9), 1),
+                      "bit_exact": bool(np.array_equal(c_host, c_dev))}
 
     # batched: the same point 16 times, and the mixed batch of 64 uneven products
     del m, da, db, dc
-    batch_same = batched(pmlib, O, 16, [(N, W)] * 16, SEED + 100, 2, 20)
-    rng = np.random.default_rng(SEED)
-    mix = [(int(2 ** rng.integers(11, 17)), int(rng.integers(128, 2049))) for _ in range(64)]
-    batch_mix = batched(pmlib, O, 64, mix, SEED + 200, 4, 20)
-    batch_mix["workload"] = "64 products, N log-uniform in 2^11..2^16, W uniform in [128, 2048], seed %d" % SEED
+    batch_same = batch_mix = None
+    if "batch" in legs:
+        tune = not args.no_autotune
+        batch_same = batched(pmlib, O, 16, [(N, W)] * 16, SEED + 100, 2, 20, tune)
+        rng = np.random.default_rng(SEED)
+        mix = [(int(2 ** rng.integers(11, 17)), int(rng.integers(128, 2049))) for _ in range(64)]
+        batch_mix = batched(pmlib, O, 64, mix, SEED + 200, 4, 20, tune)
+        batch_mix["workload"] = "64 products, N log-uniform in 2^11..2^16, W uniform in [128, 2048], seed %d" % SEED
 
-    serve = None if args.no_serve else serving(pmlib, O, (200, 1000, 5000), args.serve_seconds)
+    serve = serving(pmlib, O, (200, 1000, 5000), args.serve_seconds) if "serve" in legs else None
 
-    cpu = None if args.no_cpu else cpu_baseline(q, args.cpu_cases)
+    cpu = cpu_baseline(q, args.cpu_cases) if "cpu" in legs else None
     out = {
         "metric": METRIC, "value": round(value, 2), "unit": "mults/s", "n_gpus": 1, "steps": args.steps,
         "warmup": args.warmup, "ms_per_step": round(ms_step, 4), "higher_is_better": True, "scaling": "weak",

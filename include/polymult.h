@@ -84,6 +84,11 @@ int pm_time_device(pm_state* s, const uint32_t* d_a, const uint32_t* d_b, uint32
  * counterpart is its PM_DIAG setup benchmark (solve.cu:1245-1358). */
 int pm_probe_transfers(size_t bytes, int threads, int reps, double* out);
 
+/* HBM copy rate (GB/s, read + write) of a plain device copy kernel over
+ * `bytes` (a multiple of 16), averaged over `iters` launches. Diagnostic: the
+ * reference point for the HBM traffic counters and for achievable bandwidth. */
+int pm_probe_copy(size_t bytes, int iters, double* gbps);
+
 /* ---- stage entry points, for stage-by-stage parity tests ----
  * All on device buffers, synchronous.
  *   rns:     R[K][N] = a mod p_j

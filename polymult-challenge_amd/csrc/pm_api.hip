@@ -268,6 +268,43 @@ int pm_probe_transfers(size_t bytes, int threads, int reps, double* out) {
     return guarded([&] { pm::probe_transfers(bytes, threads, reps, out); });
 }
 
+int pm_probe_copy(size_t bytes, int iters, double* gbps) {
+    return guarded([&] {
+        if (!gbps || bytes == 0 || bytes % 16 || iters < 1) throw std::runtime_error("bad copy probe arguments");
+        void *src = nullptr, *dst = nullptr;
+        hipEvent_t e0 = nullptr, e1 = nullptr;
+        int dev = 0, cus = 0;
+        hip_check(hipGetDevice(&dev), "hipGetDevice");
+        hip_check(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev), "attribute");
+        const int grid = cus * 8;
+        try {
+            hip_check(hipMalloc(&src, bytes), "hipMalloc");
+            hip_check(hipMalloc(&dst, bytes), "hipMalloc");
+            hip_check(hipMemset(src, 1, bytes), "hipMemset");
+            hip_check(hipEventCreate(&e0), "event");
+            hip_check(hipEventCreate(&e1), "event");
+            hip_check(pm::launch_copy_probe(src, dst, bytes, grid, nullptr), "copy");
+            hip_check(hipEventRecord(e0, nullptr), "record");
+            for (int i = 0; i < iters; ++i) hip_check(pm::launch_copy_probe(src, dst, bytes, grid, nullptr), "copy");
+            hip_check(hipEventRecord(e1, nullptr), "record");
+            hip_check(hipEventSynchronize(e1), "sync");
+            float ms = 0;
+            hip_check(hipEventElapsedTime(&ms, e0, e1), "elapsed");
+            *gbps = 2.0 * (double)bytes * iters / (ms * 1e-3) / 1e9;  // read + write
+        } catch (...) {
+            if (e0) (void)hipEventDestroy(e0);
+            if (e1) (void)hipEventDestroy(e1);
+            if (src) (void)hipFree(src);
+            if (dst) (void)hipFree(dst);
+            throw;
+        }
+        (void)hipEventDestroy(e0);
+        (void)hipEventDestroy(e1);
+        (void)hipFree(src);
+        (void)hipFree(dst);
+    });
+}
+
 int pm_stage_rns(pm_state* h, const uint32_t* d_a, uint32_t* d_R) {
     return guarded([&] {
         pm::State& S = *h->s;

@@ -42,11 +42,11 @@ struct BatchConfig {
     // group_mb MiB, one group's five phases after another's, so that a
     // group's residues stay in the 256 MB last-level cache (MALL) from phase
     // to phase; 0: one group.
-    int group_mb = 192;
+    int group_mb = 96;
     int single_kernels = 1;  // a group of one product runs the single-product kernels
     // Graph branches: groups alternate over this many independent chains, so
     // one group's latency-bound kernels can overlap the next group's.
-    int branches = 1;
+    int branches = 2;
 };
 
 // Tile lists and the five phase launches for a list of products.

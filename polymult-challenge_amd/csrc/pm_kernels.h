@@ -65,6 +65,9 @@ hipError_t launch_row(const DevPlan& D, const LaunchCfg& C, uint32_t* RB, const 
 hipError_t launch_crt(const DevPlan& D, const LaunchCfg& C, const uint32_t* R, uint32_t* out, const uint2* G,
                       hipStream_t s, uint32_t first = 0, uint32_t count = 0);
 
+// dst = src, `bytes` a multiple of 16, on `grid` blocks of 256 threads.
+hipError_t launch_copy_probe(const void* src, void* dst, size_t bytes, int grid, hipStream_t s);
+
 // ---- batched products: one launch per phase over every product's tiles ----
 enum BatchPhase { PH_RNS = 0, PH_COLF = 1, PH_ROW = 2, PH_COLI = 3, PH_CRT = 4, PH_COUNT = 5 };
 

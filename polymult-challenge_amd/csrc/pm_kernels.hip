@@ -652,6 +652,13 @@ __device__ __forceinline__ void batch_task(const BatchProduct& P, u32 x, u32 y, 
     }
 }
 
+// Plain streaming copy (16 B per lane): the reference point for the HBM
+// counters and for the achievable copy bandwidth.
+__global__ void __launch_bounds__(256) k_copy_probe(const uint4* __restrict__ src, uint4* __restrict__ dst, size_t n) {
+    for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x)
+        dst[i] = src[i];
+}
+
 __global__ void k_batch_reset(u32* ctl, u32 n) {
     for (u32 i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) ctl[i] = 0;
 }
@@ -793,6 +800,13 @@ hipError_t launch_crt(const DevPlan& D, const LaunchCfg& C, const u32* R, u32* o
         case 32: hipLaunchKernelGGL((k_crt<1>), grid, block, C.crt_lds, s, D, R, out, G, count, C.crt_ys); break;
         default: return hipErrorInvalidValue;
     }
+    return hipGetLastError();
+}
+
+hipError_t launch_copy_probe(const void* src, void* dst, size_t bytes, int grid, hipStream_t s) {
+    if (bytes % 16) return hipErrorInvalidValue;
+    hipLaunchKernelGGL(k_copy_probe, dim3(grid), dim3(256), 0, s, static_cast<const uint4*>(src),
+                       static_cast<uint4*>(dst), bytes / 16);
     return hipGetLastError();
 }
 

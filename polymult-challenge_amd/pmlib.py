@@ -22,7 +22,7 @@ LIB_PATH = HERE / "libpolymult.so"
 C_SYMBOLS = (
     "pm_init", "pm_run", "pm_run_device", "pm_free", "pm_last_error", "pm_query", "pm_point_info",
     "pm_dev_alloc", "pm_dev_free", "pm_copy_to_device", "pm_copy_to_host", "pm_sync", "pm_time_device",
-    "pm_stage_rns", "pm_stage_ntt_fwd", "pm_stage_ntt_inv", "pm_stage_crt", "pm_probe_transfers",
+    "pm_stage_rns", "pm_stage_ntt_fwd", "pm_stage_ntt_inv", "pm_stage_crt", "pm_probe_transfers", "pm_probe_copy",
     "pm_batch_create", "pm_batch_free", "pm_batch_last_error", "pm_batch_product_info", "pm_batch_run",
     "pm_batch_sync", "pm_batch_time", "pm_batch_autotune", "pm_batch_configure",
     "pm_server_create", "pm_server_free", "pm_server_last_error", "pm_server_submit", "pm_server_wait",
@@ -117,6 +117,7 @@ def load() -> ctypes.CDLL:
         "pm_stage_crt": ([_vp, _vp, _vp], ctypes.c_int),
         "pm_probe_transfers": ([ctypes.c_size_t, ctypes.c_int, ctypes.c_int, ctypes.POINTER(ctypes.c_double)],
                                ctypes.c_int),
+        "pm_probe_copy": ([ctypes.c_size_t, ctypes.c_int, ctypes.POINTER(ctypes.c_double)], ctypes.c_int),
         "pm_batch_create": ([ctypes.c_uint32, _u32p, _u32p, _u32p, ctypes.POINTER(_vp)], ctypes.c_int),
         "pm_batch_free": ([_vp], None),
         "pm_batch_last_error": ([], ctypes.c_char_p),
@@ -179,6 +180,13 @@ def probe_transfers(nbytes: int, threads: int = 16, reps: int = 5) -> dict:
     keys = ["h2d_pinned", "d2h_pinned", "h2d_pageable", "d2h_pageable", "stage_1thread", "stage_threads",
             "duplex_pinned"]
     return {k: round(v, 2) for k, v in zip(keys, out)}
+
+
+def probe_copy(nbytes: int, iters: int = 10) -> float:
+    """GB/s (read + write) of a plain device copy kernel over nbytes (see pm_probe_copy)."""
+    g = ctypes.c_double()
+    _check(load().pm_probe_copy(nbytes, iters, ctypes.byref(g)))
+    return g.value
 
 
 class DeviceBuffer:
