@@ -35,6 +35,7 @@ struct DevPlan {
 // Launch geometry chosen once per plan.
 struct LaunchCfg {
     int rns_tn = 128;   // coefficients per RNS block (32, 64 or 128)
+    int rns_threads = 256;  // threads per RNS block (256 or 512)
     int rns_stride = 0; // LDS bytes per coefficient row
     size_t rns_lds = 0;
     int crt_tn = 64;    // coefficients per CRT block
@@ -87,7 +88,7 @@ struct BatchProduct {
 //   COLF x = first column, y = prime | operand << 16
 //   ROW  x = first row, y = prime
 //   COLI x = first column, y = prime
-//   CRT  x = first coefficient (64 per tile)
+//   CRT  x = first coefficient (32 per tile)
 struct BatchTask {
     uint32_t prod, x, y;
 };
@@ -96,7 +97,7 @@ struct BatchTask {
 // and the column passes, 4 for the rows and the CRT).
 constexpr int kBatchThreads[PH_COUNT] = {512, 512, 256, 512, 256};
 constexpr int kBatchRnsTile = 128;
-constexpr int kBatchCrtTile = 64;
+constexpr int kBatchCrtTile = 32;
 
 // Zeroes n words (the persistent tile counters) with a kernel, so the launch
 // can sit in a captured graph.

@@ -27,6 +27,8 @@
 // (solve.cu:947-953), here as Shoup pairs rather than Montgomery forms.
 #include "pm_kernels.h"
 
+#include <cstdlib>
+
 #include "pm_arith.h"
 
 namespace pm {
@@ -211,9 +213,14 @@ __global__ void __launch_bounds__(1024) k_col(DevPlan D, u32* __restrict__ R0, u
 }
 
 // ---------------------------------------------------------------------------
-// Row tile: rows [row0, row0 + rpb) of prime j, rpb T threads. LDS rows are
-// padded by 16 words and XOR-swizzled inside each 16-word group so that both
-// the strided pass-1 writes and the contiguous pass-2 reads spread over the
+// Row tile number row0 / rpb of prime j, rpb rows, rpb T threads. The tile
+// holds rows in mirrored pairs: its first rpb/2 rows come from the lower half
+// of the rows, the other rpb/2 are their mirrors nrows - 1 - r. The inverse
+// stages of row r read the twiddles the forward stages of row nrows - 1 - r
+// read (the mirrored entries), so a pair shares its twiddle lines in cache
+// and the table streams from HBM once instead of twice. LDS rows are padded
+// by 16 words and XOR-swizzled inside each 16-word group so that both the
+// strided pass-1 writes and the contiguous pass-2 reads spread over the
 // banks. LDS: 2 rpb (2^NB + 16) words.
 // ---------------------------------------------------------------------------
 __device__ __forceinline__ int swz(int e) { return e ^ ((e >> 4) & 15); }
@@ -226,7 +233,12 @@ __device__ __forceinline__ void row_tile(const DevPlan& D, u32* __restrict__ RB,
     constexpr int RS = (1 << NB) + (R2 > 0 ? 16 : 0);
     const bool on = tid < rpb * T;
     const int r = on ? tid / T : 0, t = tid % T;
-    const u32 row = row0 + (u32)r;
+    const u32 nrows = 1u << D.w1;
+    u32 row = row0 + (u32)r;
+    if ((rpb & 1) == 0 && (u32)rpb < nrows) {
+        const u32 h = (u32)rpb / 2, base = (row0 / (u32)rpb) * h;
+        row = (u32)r < h ? base + (u32)r : nrows - 1u - (base + (u32)r - h);
+    }
     const u32 p = D.p[j];
     const uint2* tab = D.tw + (size_t)j * D.tw_stride;
     const size_t off = (size_t)j * D.N + ((size_t)row << NB);
@@ -377,7 +389,7 @@ __device__ __forceinline__ long long slot_value(const v16i& acc, int g) {
 }
 
 // ---------------------------------------------------------------------------
-// RNS decomposition. Block = 8 waves and TN = 32 NCT consecutive coefficients,
+// RNS decomposition. Block = 4 or 8 waves and TN = 32 NCT consecutive coefficients,
 // copied verbatim (4L bytes each) into LDS. The GEMM is
 //   C[(j,t)][n] = sum_b D[(j,t)][b] (byte_b(a_n) - 128),
 // D the balanced digits of 2^(8b+32) mod p_j; sum_t 256^t C + bias_j is then
@@ -442,7 +454,7 @@ __global__ void __launch_bounds__(512) k_rns(DevPlan D, const u32* __restrict__ 
 }
 
 // ---------------------------------------------------------------------------
-// CRT. Block = 4 waves and TN = 32 NCT coefficients.
+// CRT. Block = 4 waves and TN = 32 NCT coefficients (32 by default).
 //   A: y_j = r_j G_j, stored as 4K bytes per coefficient, re-centred (^0x80).
 //   B: MFMA over (j, s) against the digit table: slot i of the result is
 //      sum_t 256^t C[4i + t]; slots 0..L give z - C0, slots L+1, L+2 give
@@ -724,15 +736,29 @@ int line_threads(int nb) { return nb >= 4 ? 1 << (nb - 4) : 1; }
 
 }  // namespace
 
+// PM_RNS_TN / PM_CRT_TN (32, 64 or 128) cap the coefficients per RNS / CRT
+// block, for tuning; the LDS limit still applies.
+static int tile_cap(const char* name) {
+    const char* v = std::getenv(name);
+    const int t = v ? std::atoi(v) : 128;
+    return (t == 32 || t == 64 || t == 128) ? t : 128;
+}
+
 LaunchCfg choose_launch(const DevPlan& D, size_t max_lds) {
     LaunchCfg C;
     const u32 nrows = 1u << D.w1, ncols = 1u << D.w2;
     C.rns_stride = 4 * D.L;
-    C.rns_tn = 128;
+    C.rns_tn = tile_cap("PM_RNS_TN");
     while (C.rns_tn > 32 && (size_t)C.rns_tn * C.rns_stride + 32 > max_lds) C.rns_tn /= 2;
     C.rns_lds = (size_t)C.rns_tn * C.rns_stride + 32;
+    // 4-wave RNS blocks by default: four blocks per CU overlap their tile
+    // loads better than two 8-wave blocks (PM_RNS_THREADS=512 for the latter).
+    if (const char* v = std::getenv("PM_RNS_THREADS")) C.rns_threads = std::atoi(v) == 512 ? 512 : 256;
     C.crt_ys = 32 * D.crt_ks + 16;
-    C.crt_tn = 128;
+    // 32 coefficients per CRT block by default: half the LDS of 64, so more
+    // blocks per CU hide the latency of the residue loads (measured faster at
+    // N=65536).
+    C.crt_tn = std::getenv("PM_CRT_TN") ? tile_cap("PM_CRT_TN") : 32;
     auto crt_need = [&](int tn) { return (size_t)tn * C.crt_ys + (size_t)tn * (D.L + 3) * 8; };
     while (C.crt_tn > 32 && crt_need(C.crt_tn) > max_lds) C.crt_tn /= 2;
     C.crt_lds = crt_need(C.crt_tn);
@@ -745,7 +771,7 @@ hipError_t launch_rns(const DevPlan& D, const LaunchCfg& C, const u32* a0, const
                       hipStream_t s, u32 first, u32 count) {
     if (count == 0) count = D.N - first;
     if (first + count > D.N) return hipErrorInvalidValue;
-    const dim3 grid((count + C.rns_tn - 1) / C.rns_tn, nops), block(512);
+    const dim3 grid((count + C.rns_tn - 1) / C.rns_tn, nops), block(C.rns_threads);
     const size_t L = (size_t)D.L;
     a0 += first * L;
     R0 += first;
