@@ -9,7 +9,7 @@ timed steps. The PCIe-inclusive rate of the host-buffer call (pm_run, the
 fherma_run replacement) is reported beside it, never as `value`.
 
     python bench.py [--gpus 1] [--steps 200] [--warmup 20] [--no-cpu] [--serve-seconds 10] [--no-serve]
-                    [--legs plain,e2e,batch,serve,cpu] [--copy-probe BYTES]
+                    [--legs plain,shard,e2e,batch,serve,cpu] [--copy-probe BYTES]
 
 Prints one JSON line.
 """
@@ -190,7 +190,7 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--serve-seconds", type=float, default=10.0)
     ap.add_argument("--no-serve", action="store_true")
-    ap.add_argument("--legs", default="plain,e2e,batch,serve,cpu",
+    ap.add_argument("--legs", default="plain,shard,e2e,batch,serve,cpu",
                     help="comma list of parts to run (plain is always run: it gives value)")
     ap.add_argument("--no-autotune", action="store_true", help="batched legs on the default configuration")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="BYTES",
@@ -255,6 +255,24 @@ def main():
     per_kernel = {k: {"ms": round(kms[k], 4), "GBps": round(kb[k] / (kms[k] * 1e-3) / 1e9, 1),
                       "Gmodmul_per_s": round(mm[k] / (kms[k] * 1e-3) / 1e9, 1)} for k in KERNELS}
 
+    # prime shards: what one GPU of an 8-GPU prime-sharded multiply runs
+    shard = None
+    if "shard" in legs:
+        G = 8
+        t = m.time_shard(da, db, dc, G, 20)
+        m.run_device_sharded(da, db, dc, G)
+        m.sync()
+        part = (L + 3) * N * 8
+        chain = sum(t[k] for k in ("rns", "col_fwd", "row_fused", "col_inv", "crt_partial"))
+        shard = {"shards": G, "bounds": m.shard_bounds(G),
+                 "shard0_ms": {k: round(v, 4) for k, v in t.items() if k != "sharded_multiply"},
+                 "shard0_chain_ms": round(chain, 4),
+                 "all_shards_here_ms": round(t["sharded_multiply"], 4),
+                 "partial_bytes_per_shard": part,
+                 "bit_exact_vs_plain": bool(np.array_equal(dc.download((N, L)), c_dev)),
+                 "note": "shards run one after another on this GPU; on G GPUs each runs its chain at once, then a "
+                         "reduce-scatter of the partials and the combine on N/G coefficients each"}
+
     # end to end from pageable host buffers (the fherma_run replacement)
     pcie_bytes = 12 * N * L
     end_to_end = None
@@ -294,7 +312,7 @@ def main():
                    "operands": "mixed per coefficient: 1/2 uniform [0,q), 1/4 q-1, 1/4 < 2^32", "seed": SEED},
         "bit_exact": correct,
         "pct_hbm_peak": round(100 * pcie_bytes / (ms_step * 1e-3) / (HBM_PEAK_GBS * 1e9), 3),
-        "roofline": roofline, "kernels": per_kernel, "end_to_end": end_to_end,
+        "roofline": roofline, "kernels": per_kernel, "end_to_end": end_to_end, "prime_shard": shard,
         "batched_same_point": batch_same, "batched_mixed": batch_mix, "serving": serve, "cpu_baseline": cpu,
     }
     print(json.dumps(out))

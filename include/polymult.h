@@ -84,6 +84,24 @@ int pm_time_device(pm_state* s, const uint32_t* d_a, const uint32_t* d_b, uint32
  * counterpart is its PM_DIAG setup benchmark (solve.cu:1245-1358). */
 int pm_probe_transfers(size_t bytes, int threads, int reps, double* out);
 
+/* ---- prime shards: the multi-GPU split, shown on one GPU ----
+ * No counterpart in the reference (single GPU). The K primes split into
+ * `shards` ranges of whole groups of 8 primes (1 <= shards <= ceil(K/8));
+ * bounds[0..shards] receives them. A shard runs RNS, the NTT passes and the
+ * linear part of the CRT (int64 slot partials, (L + 3) x N) for its primes
+ * only, as one GPU of a sharded multiply would; the combine sums the partials
+ * (where GPUs would reduce-scatter them) and finishes the CRT.
+ * pm_run_device_sharded runs the shards one after another on this GPU and
+ * gives the same c as pm_run_device. pm_time_shard times, averaged over
+ * `iters`: ms[0] the whole sharded multiply here, ms[1..5] shard 0's RNS,
+ * forward columns, fused rows, inverse columns and CRT partial, ms[6] the
+ * combine. */
+int pm_shard_bounds(pm_state* s, int shards, uint32_t* bounds);
+int pm_run_device_sharded(pm_state* s, const uint32_t* d_a, const uint32_t* d_b, uint32_t* d_c, int shards,
+                          void* stream);
+int pm_time_shard(pm_state* s, const uint32_t* d_a, const uint32_t* d_b, uint32_t* d_c, int shards, int iters,
+                  float* ms);
+
 /* HBM copy rate (GB/s, read + write) of a plain device copy kernel over
  * `bytes` (a multiple of 16), averaged over `iters` launches. Diagnostic: the
  * reference point for the HBM traffic counters and for achievable bandwidth. */

@@ -3,6 +3,7 @@
 // entry points and device-buffer helpers.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cstring>
 #include <memory>
 #include <stdexcept>
@@ -90,6 +91,7 @@ void State::upload_plan() {
     D.crt_frag = reinterpret_cast<const v4i*>(at(o_cf));
     D.crt_rt = H.crt_rt;
     D.crt_ks = H.crt_ks;
+    D.crt_ks_full = H.crt_ks;
     D.G = reinterpret_cast<const uint2*>(at(o_G));
     D.Gplain = reinterpret_cast<const uint2*>(at(o_Gp));
     D.C0 = reinterpret_cast<const uint32_t*>(at(o_C0));
@@ -134,8 +136,74 @@ hipError_t State::enqueue_multiply(const uint32_t* a, const uint32_t* b, uint32_
     return launch_crt(dev, cfg, d_R[1], c, dev.G, st);
 }
 
+std::vector<uint32_t> State::shard_bounds(int shards) const {
+    const uint32_t K = (uint32_t)dev.K, groups = (K + 7) / 8;
+    if (shards < 1 || (uint32_t)shards > groups)
+        throw std::runtime_error("shards must be between 1 and " + std::to_string(groups) +
+                                 " (whole groups of 8 primes) at this point");
+    std::vector<uint32_t> b(shards + 1);
+    // larger shards first: shard 0 is a largest one
+    for (int g = 0; g <= shards; ++g)
+        b[g] = std::min(K, 8 * (uint32_t)(((uint64_t)groups * g + shards - 1) / shards));
+    return b;
+}
+
+DevPlan State::shard_view(uint32_t j0, uint32_t cnt) const {
+    DevPlan V = dev;
+    V.K = (int)cnt;
+    V.p += j0;
+    V.pinv += j0;
+    V.tw += (size_t)j0 * dev.tw_stride;
+    V.rns_frag += (size_t)(j0 / 8) * dev.rns_ks * 64;  // row tile = 8 primes
+    V.rns_bias += j0;
+    V.rns_rt = (4 * (int)cnt + 31) / 32;
+    V.crt_frag += (size_t)(j0 / 8) * 64;  // k-step = 8 primes; rows keep the full stride
+    V.crt_ks = ((int)cnt + 7) / 8;
+    V.crt_ks_full = dev.crt_ks_full;
+    V.G += j0;
+    V.Gplain += j0;
+    return V;
+}
+
+// What one GPU of a prime-sharded multiply runs: its primes through RNS, the
+// three NTT passes and the CRT's linear part.
+hipError_t State::enqueue_shard(const DevPlan& V, const LaunchCfg& C, uint32_t j0, const uint32_t* a,
+                                const uint32_t* b, long long* part, hipStream_t st) {
+    uint32_t* R0 = d_R[0] + (size_t)j0 * dev.N;
+    uint32_t* R1 = d_R[1] + (size_t)j0 * dev.N;
+    hipError_t e;
+    if ((e = launch_rns(V, C, a, b, R0, R1, 2, st)) != hipSuccess) return e;
+    if ((e = launch_col(V, C, R0, R1, 2, false, st)) != hipSuccess) return e;
+    if ((e = launch_row(V, C, R1, R0, ROW_FUSED, st)) != hipSuccess) return e;
+    if ((e = launch_col(V, C, R1, nullptr, 1, true, st)) != hipSuccess) return e;
+    return launch_crt_partial(V, C, R1, part, st);
+}
+
+void State::enqueue_sharded(const uint32_t* a, const uint32_t* b, uint32_t* c, int shards, hipStream_t st) {
+    const std::vector<uint32_t> bd = shard_bounds(shards);
+    const size_t stride = (size_t)(dev.L + 3) * dev.N;  // int64 slots per shard
+    const size_t need = (size_t)shards * stride * 8;
+    if (need > part_bytes) {
+        hip_check(hipStreamSynchronize(st), "sync");
+        if (d_part) hip_check(hipFree(d_part), "hipFree");
+        d_part = nullptr;
+        part_bytes = 0;
+        hip_check(hipMalloc(&d_part, need), "hipMalloc shard partials");
+        part_bytes = need;
+    }
+    int max_lds = 0;
+    hip_check(hipDeviceGetAttribute(&max_lds, hipDeviceAttributeMaxSharedMemoryPerBlock, device), "attribute");
+    for (int g = 0; g < shards; ++g) {
+        const DevPlan V = shard_view(bd[g], bd[g + 1] - bd[g]);
+        const LaunchCfg C = choose_launch(V, std::min<size_t>((size_t)max_lds, 64 * 1024));
+        hip_check(enqueue_shard(V, C, bd[g], a, b, d_part + (size_t)g * stride, st), "shard");
+    }
+    hip_check(launch_crt_combine(dev, cfg, d_part, shards, stride, c, st), "combine");
+}
+
 State::~State() {
     if (stream) (void)hipStreamSynchronize(stream);
+    if (d_part) (void)hipFree(d_part);
     pipeline_free();
     for (int i = 0; i < 2; ++i) {
         if (d_R[i]) (void)hipFree(d_R[i]);
@@ -261,6 +329,70 @@ int pm_time_device(pm_state* h, const uint32_t* d_a, const uint32_t* d_b, uint32
         }
         for (auto& e : ev) (void)hipEventDestroy(e);
         for (int k = 0; k < NE; ++k) ms[k] = (float)(acc[k] / (iters > 0 ? iters : 1));
+    });
+}
+
+int pm_shard_bounds(pm_state* h, int shards, uint32_t* bounds) {
+    return guarded([&] {
+        if (!h || !bounds) throw std::runtime_error("null argument");
+        const std::vector<uint32_t> b = h->s->shard_bounds(shards);
+        std::memcpy(bounds, b.data(), b.size() * 4);
+    });
+}
+
+int pm_run_device_sharded(pm_state* h, const uint32_t* d_a, const uint32_t* d_b, uint32_t* d_c, int shards,
+                          void* stream) {
+    return guarded([&] {
+        pm::State& S = *h->s;
+        S.enqueue_sharded(d_a, d_b, d_c, shards, stream ? static_cast<hipStream_t>(stream) : S.stream);
+    });
+}
+
+int pm_time_shard(pm_state* h, const uint32_t* d_a, const uint32_t* d_b, uint32_t* d_c, int shards, int iters,
+                  float* ms) {
+    return guarded([&] {
+        pm::State& S = *h->s;
+        hipStream_t st = S.stream;
+        S.enqueue_sharded(d_a, d_b, d_c, shards, st);  // sizes the partials, warms up
+        hip_check(hipStreamSynchronize(st), "sync");
+        const std::vector<uint32_t> bd = S.shard_bounds(shards);
+        int max_lds = 0;
+        hip_check(hipDeviceGetAttribute(&max_lds, hipDeviceAttributeMaxSharedMemoryPerBlock, S.device), "attribute");
+        // shard 0 is a largest one
+        const pm::DevPlan V = S.shard_view(bd[0], bd[1] - bd[0]);
+        const pm::LaunchCfg C = pm::choose_launch(V, std::min<size_t>((size_t)max_lds, 64 * 1024));
+        const size_t stride = (size_t)(S.dev.L + 3) * S.dev.N;
+        uint32_t* R0 = S.d_R[0];
+        uint32_t* R1 = S.d_R[1];
+        constexpr int NE = 8;
+        hipEvent_t ev[NE];
+        for (auto& e : ev) hip_check(hipEventCreate(&e), "event");
+        double acc[NE] = {0};
+        for (int it = 0; it < iters; ++it) {
+            hip_check(hipEventRecord(ev[0], st), "record");
+            S.enqueue_sharded(d_a, d_b, d_c, shards, st);
+            hip_check(hipEventRecord(ev[1], st), "record");
+            hip_check(pm::launch_rns(V, C, d_a, d_b, R0, R1, 2, st), "rns");
+            hip_check(hipEventRecord(ev[2], st), "record");
+            hip_check(pm::launch_col(V, C, R0, R1, 2, false, st), "col");
+            hip_check(hipEventRecord(ev[3], st), "record");
+            hip_check(pm::launch_row(V, C, R1, R0, pm::ROW_FUSED, st), "row");
+            hip_check(hipEventRecord(ev[4], st), "record");
+            hip_check(pm::launch_col(V, C, R1, nullptr, 1, true, st), "col inv");
+            hip_check(hipEventRecord(ev[5], st), "record");
+            hip_check(pm::launch_crt_partial(V, C, R1, S.d_part, st), "crt partial");
+            hip_check(hipEventRecord(ev[6], st), "record");
+            hip_check(pm::launch_crt_combine(S.dev, S.cfg, S.d_part, shards, stride, d_c, st), "combine");
+            hip_check(hipEventRecord(ev[7], st), "record");
+            hip_check(hipEventSynchronize(ev[7]), "sync");
+            for (int k = 1; k < NE; ++k) {
+                float t = 0;
+                hip_check(hipEventElapsedTime(&t, ev[k - 1], ev[k]), "elapsed");
+                acc[k - 1] += t;
+            }
+        }
+        for (auto& e : ev) (void)hipEventDestroy(e);
+        for (int k = 0; k < NE - 1; ++k) ms[k] = (float)(acc[k] / (iters > 0 ? iters : 1));
     });
 }
 

@@ -123,6 +123,7 @@ void Catalog::build(uint32_t count, const uint32_t* N, const uint32_t* W, const 
         D.crt_frag = reinterpret_cast<const v4i*>(at(offs[i].cf));
         D.crt_rt = H.crt_rt;
         D.crt_ks = H.crt_ks;
+        D.crt_ks_full = H.crt_ks;
         D.G = reinterpret_cast<const uint2*>(at(offs[i].G));
         D.Gplain = reinterpret_cast<const uint2*>(at(offs[i].Gp));
         D.C0 = reinterpret_cast<const uint32_t*>(at(offs[i].C0));

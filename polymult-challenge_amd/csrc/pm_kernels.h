@@ -22,8 +22,9 @@ struct DevPlan {
     const uint64_t* rns_bias;
     int rns_rt, rns_ks;
     uint32_t magic_L;      // ceil(2^32 / L) for L >= 2: idx / L = umulhi(idx, magic_L) for idx < 2^20
-    const v4i* crt_frag;   // [crt_rt][crt_ks][64]
-    int crt_rt, crt_ks;
+    const v4i* crt_frag;   // [crt_rt][crt_ks_full][64]
+    int crt_rt, crt_ks;    // crt_ks: k-steps (8 primes each) this plan covers
+    int crt_ks_full;       // k-steps per row tile of the table (crt_ks unless a prime shard)
     const uint2* G;        // [K]
     const uint2* Gplain;   // [K]
     const uint32_t* C0;    // [L + 2]
@@ -65,6 +66,13 @@ hipError_t launch_row(const DevPlan& D, const LaunchCfg& C, uint32_t* RB, const 
 // out[n][L] = CRT of R[j][n] (scaled by G) mod q, for n in [first, first + count).
 hipError_t launch_crt(const DevPlan& D, const LaunchCfg& C, const uint32_t* R, uint32_t* out, const uint2* G,
                       hipStream_t s, uint32_t first = 0, uint32_t count = 0);
+
+// Prime shards: D a shard's view (its primes, whole k-steps). P[slot][n]
+// int64, L + 3 slots; the combine sums `shards` such arrays (shard_stride
+// apart) with the full plan D and finishes the CRT into out[N][L].
+hipError_t launch_crt_partial(const DevPlan& D, const LaunchCfg& C, const uint32_t* R, long long* P, hipStream_t s);
+hipError_t launch_crt_combine(const DevPlan& D, const LaunchCfg& C, const long long* P, int shards, size_t shard_stride,
+                              uint32_t* out, hipStream_t s);
 
 // dst = src, `bytes` a multiple of 16, on `grid` blocks of 256 threads.
 hipError_t launch_copy_probe(const void* src, void* dst, size_t bytes, int grid, hipStream_t s);

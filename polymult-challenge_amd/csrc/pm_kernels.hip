@@ -463,50 +463,52 @@ __global__ void __launch_bounds__(512) k_rns(DevPlan D, const u32* __restrict__ 
 //      z = sum_i P_i 2^(32 i) + C0 + T[v] into L + 2 limbs, fold with
 //      2^W = delta until z < 2^W, subtract q once if needed.
 //   D: coalesced store of the block's contiguous [TN][L] output.
+// A and B are linear in the primes: over a subset of the primes (a prime
+// shard, whole MFMA k-steps of 8 primes) they give that shard's share of
+// every slot, and the shares add up to the full slots (k_crt_partial,
+// k_crt_combine).
 // ---------------------------------------------------------------------------
-template <int NCT>
-__device__ __forceinline__ void crt_tile(const DevPlan& D, const u32* __restrict__ R, u32* __restrict__ out,
-                                         const uint2* __restrict__ G, u32 n0, u32 count, int ys, unsigned char* lds8,
-                                         int tid, int nthreads) {
-    constexpr int TN = 32 * NCT;
-    const int L = D.L, K = D.K, nslot = L + 3, ZL = L + 2;
-    const u32 N = D.N;
-    unsigned char* Y = lds8;                                           // [TN][ys]
-    // slot-major, so lanes (consecutive coefficients) touch consecutive words
-    long long* Pp = reinterpret_cast<long long*>(lds8 + (size_t)TN * ys);  // [nslot][TN]
-    const int cnt = (int)min((u32)TN, count - n0);
 
-    // y bytes: thread (c, r) covers primes j = r, r + JS, ... for coefficient c,
-    // eight loads in flight per batch (lanes sweep c, so loads coalesce).
-    {
-        const int kp = 8 * D.crt_ks;
-        const int JS = nthreads / TN;
-        const int c = tid % TN, r = tid / TN;
-        const bool live = c < cnt;
-        const u32* Rc = R + n0 + c;
-        for (int j0 = r; j0 < kp; j0 += 8 * JS) {
-            u32 v[8];
+// Phase A for coefficients [n0, n0 + cnt): thread (c, r) covers primes j = r,
+// r + JS, ... of coefficient c, eight loads in flight per batch (lanes sweep
+// c, so loads coalesce).
+template <int TN>
+__device__ __forceinline__ void crt_y(const DevPlan& D, const u32* __restrict__ R, const uint2* __restrict__ G, u32 n0,
+                                      int cnt, int ys, unsigned char* Y, int tid, int nthreads) {
+    const int K = D.K;
+    const u32 N = D.N;
+    const int kp = 8 * D.crt_ks;
+    const int JS = nthreads / TN;
+    const int c = tid % TN, r = tid / TN;
+    const bool live = c < cnt;
+    const u32* Rc = R + n0 + c;
+    for (int j0 = r; j0 < kp; j0 += 8 * JS) {
+        u32 v[8];
 #pragma unroll
-            for (int i = 0; i < 8; ++i) {
-                const int j = j0 + i * JS;
-                v[i] = (live && j < K) ? Rc[(size_t)j * N] : 0u;
-            }
+        for (int i = 0; i < 8; ++i) {
+            const int j = j0 + i * JS;
+            v[i] = (live && j < K) ? Rc[(size_t)j * N] : 0u;
+        }
 #pragma unroll
-            for (int i = 0; i < 8; ++i) {
-                const int j = j0 + i * JS;
-                if (j < kp) {
-                    u32 y = 0;
-                    if (live && j < K) {
-                        const uint2 g = G[j];
-                        y = shoup_mul(v[i], g.x, g.y, D.p[j]);
-                    }
-                    *reinterpret_cast<u32*>(Y + c * ys + 4 * j) = y ^ 0x80808080u;
+        for (int i = 0; i < 8; ++i) {
+            const int j = j0 + i * JS;
+            if (j < kp) {
+                u32 y = 0;
+                if (live && j < K) {
+                    const uint2 g = G[j];
+                    y = shoup_mul(v[i], g.x, g.y, D.p[j]);
                 }
+                *reinterpret_cast<u32*>(Y + c * ys + 4 * j) = y ^ 0x80808080u;
             }
         }
     }
-    __syncthreads();
+}
 
+// Phase B: put(slot, c, value) for every slot < L + 3 and column c < TN.
+template <int NCT, class PUT>
+__device__ __forceinline__ void crt_gemm(const DevPlan& D, const unsigned char* Y, int ys, int tid, int nthreads,
+                                         PUT put) {
+    const int nslot = D.L + 3;
     const int lane = tid & 63, wave = tid >> 6, nw = nthreads >> 6;
     const int half = lane >> 5, col = lane & 31;
     auto bfrag = [&](int ct, int ks) {
@@ -516,24 +518,28 @@ __device__ __forceinline__ void crt_tile(const DevPlan& D, const u32* __restrict
         v16i acc[NCT];
 #pragma unroll
         for (int ct = 0; ct < NCT; ++ct) acc[ct] = (v16i){0};
-        gemm_rows<NCT>(acc, D.crt_frag + (size_t)rt * D.crt_ks * 64 + lane, D.crt_ks, bfrag);
+        gemm_rows<NCT>(acc, D.crt_frag + (size_t)rt * D.crt_ks_full * 64 + lane, D.crt_ks, bfrag);
 #pragma unroll
         for (int g = 0; g < 4; ++g) {
             const int slot = 8 * rt + 2 * g + half;
             if (slot >= nslot) continue;
 #pragma unroll
-            for (int ct = 0; ct < NCT; ++ct) Pp[(size_t)slot * TN + 32 * ct + col] = slot_value(acc[ct], g);
+            for (int ct = 0; ct < NCT; ++ct) put(slot, 32 * ct + col, slot_value(acc[ct], g));
         }
     }
-    __syncthreads();
+}
 
-    u32* Z = reinterpret_cast<u32*>(Y);  // [TN][ZL], reuses the y bytes
+// Phases C and D for coefficients [n0, n0 + cnt); get(slot, c) returns the
+// slot value of coefficient c. Z: LDS [cnt][L + 2] words.
+template <class GET>
+__device__ __forceinline__ void crt_finish(const DevPlan& D, GET get, u32* __restrict__ out, u32 n0, int cnt, u32* Z,
+                                           int tid, int nthreads) {
+    const int L = D.L, K = D.K, ZL = L + 2;
     const int W = D.W, wl = W >> 5, wb = W & 31;
     const u32* C0 = D.C0;
     for (int c = tid; c < cnt; c += nthreads) {
-        const long long* P = Pp + c;  // slot i at P[i * TN]
         u32* z = Z + (size_t)c * ZL;
-        const u64 F = (u64)P[(L + 1) * TN] + ((u64)P[(L + 2) * TN] << 32) + (u64)D.F0;
+        const u64 F = (u64)get(L + 1, c) + ((u64)get(L + 2, c) << 32) + (u64)D.F0;
         // v <= K for every valid input; the clamp only keeps a corrupted sum
         // from indexing past T.
         const u32 v = min((u32)((F + (1ull << 47)) >> 48), (u32)K);
@@ -541,12 +547,12 @@ __device__ __forceinline__ void crt_tile(const DevPlan& D, const u32* __restrict
         long long carry = 0;
 #pragma unroll 8
         for (int i = 0; i < L; ++i) {
-            const long long s = carry + P[i * TN] + (long long)C0[i] + (long long)Tv[i];
+            const long long s = carry + get(i, c) + (long long)C0[i] + (long long)Tv[i];
             z[i] = (u32)s;
             carry = s >> 32;  // arithmetic: the running sum may be negative
         }
         {
-            long long s = carry + P[L * TN] + (long long)C0[L];
+            long long s = carry + get(L, c) + (long long)C0[L];
             z[L] = (u32)s;
             s = (s >> 32) + (long long)C0[L + 1];
             z[L + 1] = (u32)s;
@@ -595,6 +601,68 @@ __device__ __forceinline__ void crt_tile(const DevPlan& D, const u32* __restrict
         const int c = L == 1 ? idx : (int)__umulhi((u32)idx, D.magic_L);  // idx / L
         dst[idx] = Z[(size_t)c * ZL + (idx - c * L)];
     }
+}
+
+template <int NCT>
+__device__ __forceinline__ void crt_tile(const DevPlan& D, const u32* __restrict__ R, u32* __restrict__ out,
+                                         const uint2* __restrict__ G, u32 n0, u32 count, int ys, unsigned char* lds8,
+                                         int tid, int nthreads) {
+    constexpr int TN = 32 * NCT;
+    const int cnt = (int)min((u32)TN, count - n0);
+    unsigned char* Y = lds8;  // [TN][ys]
+    // slot-major, so lanes (consecutive coefficients) touch consecutive words
+    long long* Pp = reinterpret_cast<long long*>(lds8 + (size_t)TN * ys);  // [nslot][TN]
+    crt_y<TN>(D, R, G, n0, cnt, ys, Y, tid, nthreads);
+    __syncthreads();
+    crt_gemm<NCT>(D, Y, ys, tid, nthreads, [&](int slot, int c, long long v) { Pp[(size_t)slot * TN + c] = v; });
+    __syncthreads();
+    crt_finish(D, [&](int slot, int c) { return Pp[(size_t)slot * TN + c]; }, out, n0, cnt,
+               reinterpret_cast<u32*>(Y) /* reuses the y bytes */, tid, nthreads);
+}
+
+// One prime shard's share of every slot: P[slot][n] (int64), for the
+// coefficients [32 NCT blockIdx.x, ...) below count. D is the shard's view.
+template <int NCT>
+__global__ void __launch_bounds__(256) k_crt_partial(DevPlan D, const u32* __restrict__ R, const uint2* __restrict__ G,
+                                                      long long* __restrict__ P, u32 count, int ys) {
+    extern __shared__ __align__(16) unsigned char lds8[];
+    constexpr int TN = 32 * NCT;
+    const u32 n0 = blockIdx.x * (u32)TN;
+    const int cnt = (int)min((u32)TN, count - n0);
+    const int tid = (int)threadIdx.x;
+    crt_y<TN>(D, R, G, n0, cnt, ys, lds8, tid, (int)blockDim.x);
+    __syncthreads();
+    const u32 N = D.N;
+    crt_gemm<NCT>(D, lds8, ys, tid, (int)blockDim.x, [&](int slot, int c, long long v) {
+        if (c < cnt) P[(size_t)slot * N + n0 + c] = v;
+    });
+}
+
+// Sums the shards' slots and finishes: c = CRT result mod q. D is the full
+// plan. The block first sums its coefficients' slots over the shards into LDS
+// (all threads, coalesced), then finishes one coefficient per thread.
+template <int NCT>
+__global__ void __launch_bounds__(256) k_crt_combine(DevPlan D, const long long* __restrict__ P, int shards,
+                                                      size_t shard_stride, u32* __restrict__ out, u32 count) {
+    extern __shared__ __align__(16) unsigned char lds8[];
+    constexpr int TN = 32 * NCT;
+    const int nslot = D.L + 3;
+    long long* S = reinterpret_cast<long long*>(lds8);                   // [nslot][TN]
+    u32* Z = reinterpret_cast<u32*>(lds8 + (size_t)nslot * TN * 8);      // [TN][L + 2]
+    const u32 n0 = blockIdx.x * (u32)TN;
+    const int cnt = (int)min((u32)TN, count - n0);
+    const u32 N = D.N;
+    const int tid = (int)threadIdx.x;
+    for (int idx = tid; idx < nslot * TN; idx += (int)blockDim.x) {
+        const int slot = idx / TN, c = idx % TN;
+        long long s = 0;
+        if (c < cnt)
+            for (int g = 0; g < shards; ++g) s += P[(size_t)g * shard_stride + (size_t)slot * N + n0 + c];
+        S[idx] = s;
+    }
+    __syncthreads();
+    crt_finish(D, [&](int slot, int c) { return S[(size_t)slot * TN + c]; }, out, n0, cnt, Z, tid,
+               (int)blockDim.x);
 }
 
 template <int NCT>
@@ -824,6 +892,31 @@ hipError_t launch_crt(const DevPlan& D, const LaunchCfg& C, const u32* R, u32* o
         case 128: hipLaunchKernelGGL((k_crt<4>), grid, block, C.crt_lds, s, D, R, out, G, count, C.crt_ys); break;
         case 64: hipLaunchKernelGGL((k_crt<2>), grid, block, C.crt_lds, s, D, R, out, G, count, C.crt_ys); break;
         case 32: hipLaunchKernelGGL((k_crt<1>), grid, block, C.crt_lds, s, D, R, out, G, count, C.crt_ys); break;
+        default: return hipErrorInvalidValue;
+    }
+    return hipGetLastError();
+}
+
+hipError_t launch_crt_partial(const DevPlan& D, const LaunchCfg& C, const u32* R, long long* P, hipStream_t s) {
+    const dim3 grid((D.N + C.crt_tn - 1) / C.crt_tn), block(256);
+    const size_t lds = (size_t)C.crt_tn * C.crt_ys;
+    switch (C.crt_tn) {
+        case 128: hipLaunchKernelGGL((k_crt_partial<4>), grid, block, lds, s, D, R, D.G, P, D.N, C.crt_ys); break;
+        case 64: hipLaunchKernelGGL((k_crt_partial<2>), grid, block, lds, s, D, R, D.G, P, D.N, C.crt_ys); break;
+        case 32: hipLaunchKernelGGL((k_crt_partial<1>), grid, block, lds, s, D, R, D.G, P, D.N, C.crt_ys); break;
+        default: return hipErrorInvalidValue;
+    }
+    return hipGetLastError();
+}
+
+hipError_t launch_crt_combine(const DevPlan& D, const LaunchCfg& C, const long long* P, int shards, size_t shard_stride,
+                              u32* out, hipStream_t s) {
+    const dim3 grid((D.N + C.crt_tn - 1) / C.crt_tn), block(256);
+    const size_t lds = (size_t)C.crt_tn * ((D.L + 3) * 8 + (D.L + 2) * 4);
+    switch (C.crt_tn) {
+        case 128: hipLaunchKernelGGL((k_crt_combine<4>), grid, block, lds, s, D, P, shards, shard_stride, out, D.N); break;
+        case 64: hipLaunchKernelGGL((k_crt_combine<2>), grid, block, lds, s, D, P, shards, shard_stride, out, D.N); break;
+        case 32: hipLaunchKernelGGL((k_crt_combine<1>), grid, block, lds, s, D, P, shards, shard_stride, out, D.N); break;
         default: return hipErrorInvalidValue;
     }
     return hipGetLastError();

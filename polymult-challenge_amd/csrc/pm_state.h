@@ -5,6 +5,7 @@
 
 #include <cstddef>
 #include <cstdint>
+#include <vector>
 
 #include "pm_kernels.h"
 #include "pm_plan.h"
@@ -27,11 +28,22 @@ struct State {
     size_t coeff_words = 0;                  // N * L
     size_t device_bytes = 0;
     Pipeline* pipe = nullptr;
+    long long* d_part = nullptr;  // prime-shard CRT partials, grown on demand
+    size_t part_bytes = 0;
 
     void init(uint32_t N, uint32_t W, uint32_t L, const uint32_t* q, size_t q_limbs);
     void upload_plan();
     // The whole multiply on device buffers, enqueued on st.
     hipError_t enqueue_multiply(const uint32_t* a, const uint32_t* b, uint32_t* c, hipStream_t st);
+
+    // Prime shards (the multi-GPU split, run here one shard after another):
+    // boundaries j_0 = 0 < j_1 < ... < j_G = K on whole MFMA k-steps (8
+    // primes), a shard's view of the plan, and the sharded multiply.
+    std::vector<uint32_t> shard_bounds(int shards) const;
+    DevPlan shard_view(uint32_t j0, uint32_t cnt) const;
+    hipError_t enqueue_shard(const DevPlan& V, const LaunchCfg& C, uint32_t j0, const uint32_t* a, const uint32_t* b,
+                             long long* part, hipStream_t st);
+    void enqueue_sharded(const uint32_t* a, const uint32_t* b, uint32_t* c, int shards, hipStream_t st);
 
     void pipeline_init();
     void pipeline_free();

@@ -22,6 +22,7 @@ LIB_PATH = HERE / "libpolymult.so"
 C_SYMBOLS = (
     "pm_init", "pm_run", "pm_run_device", "pm_free", "pm_last_error", "pm_query", "pm_point_info",
     "pm_dev_alloc", "pm_dev_free", "pm_copy_to_device", "pm_copy_to_host", "pm_sync", "pm_time_device",
+    "pm_shard_bounds", "pm_run_device_sharded", "pm_time_shard",
     "pm_stage_rns", "pm_stage_ntt_fwd", "pm_stage_ntt_inv", "pm_stage_crt", "pm_probe_transfers", "pm_probe_copy",
     "pm_batch_create", "pm_batch_free", "pm_batch_last_error", "pm_batch_product_info", "pm_batch_run",
     "pm_batch_sync", "pm_batch_time", "pm_batch_autotune", "pm_batch_configure",
@@ -118,6 +119,10 @@ def load() -> ctypes.CDLL:
         "pm_probe_transfers": ([ctypes.c_size_t, ctypes.c_int, ctypes.c_int, ctypes.POINTER(ctypes.c_double)],
                                ctypes.c_int),
         "pm_probe_copy": ([ctypes.c_size_t, ctypes.c_int, ctypes.POINTER(ctypes.c_double)], ctypes.c_int),
+        "pm_shard_bounds": ([_vp, ctypes.c_int, _u32p], ctypes.c_int),
+        "pm_run_device_sharded": ([_vp, _vp, _vp, _vp, ctypes.c_int, _vp], ctypes.c_int),
+        "pm_time_shard": ([_vp, _vp, _vp, _vp, ctypes.c_int, ctypes.c_int, ctypes.POINTER(ctypes.c_float)],
+                          ctypes.c_int),
         "pm_batch_create": ([ctypes.c_uint32, _u32p, _u32p, _u32p, ctypes.POINTER(_vp)], ctypes.c_int),
         "pm_batch_free": ([_vp], None),
         "pm_batch_last_error": ([], ctypes.c_char_p),
@@ -276,6 +281,21 @@ class Multiplier:
         ms = (ctypes.c_float * 6)()
         _check(load().pm_time_device(self._h, da.ptr, db.ptr, dc.ptr, iters, ms))
         return list(ms)
+
+    # -- prime shards ---------------------------------------------------------
+    def shard_bounds(self, shards: int) -> list[int]:
+        b = np.zeros(shards + 1, dtype=np.uint32)
+        _check(load().pm_shard_bounds(self._h, shards, _ptr(b)))
+        return [int(x) for x in b]
+
+    def run_device_sharded(self, da: DeviceBuffer, db: DeviceBuffer, dc: DeviceBuffer, shards: int, stream=None):
+        _check(load().pm_run_device_sharded(self._h, da.ptr, db.ptr, dc.ptr, shards, stream))
+
+    def time_shard(self, da: DeviceBuffer, db: DeviceBuffer, dc: DeviceBuffer, shards: int, iters: int) -> dict:
+        ms = (ctypes.c_float * 7)()
+        _check(load().pm_time_shard(self._h, da.ptr, db.ptr, dc.ptr, shards, iters, ms))
+        keys = ["sharded_multiply", "rns", "col_fwd", "row_fused", "col_inv", "crt_partial", "combine"]
+        return dict(zip(keys, list(ms)))
 
     # -- stage entry points ---------------------------------------------------
     def stage_rns(self, a: np.ndarray) -> np.ndarray:
