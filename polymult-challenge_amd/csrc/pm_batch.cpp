@@ -166,7 +166,7 @@ void Runner::plan(const BatchConfig& c) {
         lds[PH_CRT] = std::max(lds[PH_CRT], (size_t)kBatchCrtTile * B.crt_ys + (size_t)kBatchCrtTile * (D.L + 3) * 8);
         const size_t col_l = D.w1 > 4 ? ((size_t)1 << D.w1) * B.col_tc * 4 : 0;
         lds[PH_COLF] = lds[PH_COLI] = std::max(lds[PH_COLF], col_l);
-        const size_t row_l = D.w2 > 4 ? 2 * (size_t)B.row_rpb * (((size_t)1 << D.w2) + 16) * 4 : 0;
+        const size_t row_l = D.w2 > 4 ? (size_t)B.row_rpb * (((size_t)1 << D.w2) + 16) * 4 : 0;
         lds[PH_ROW] = std::max(lds[PH_ROW], row_l);
     }
     for (size_t l : lds)

@@ -221,7 +221,7 @@ __global__ void __launch_bounds__(1024) k_col(DevPlan D, u32* __restrict__ R0, u
 // and the table streams from HBM once instead of twice. LDS rows are padded
 // by 16 words and XOR-swizzled inside each 16-word group so that both the
 // strided pass-1 writes and the contiguous pass-2 reads spread over the
-// banks. LDS: 2 rpb (2^NB + 16) words.
+// banks. LDS: rpb (2^NB + 16) words (one row buffer per row).
 // ---------------------------------------------------------------------------
 __device__ __forceinline__ int swz(int e) { return e ^ ((e >> 4) & 15); }
 
@@ -245,7 +245,6 @@ __device__ __forceinline__ void row_tile(const DevPlan& D, u32* __restrict__ RB,
     u32* rb = RB + off;
     const int s0 = D.w1;
     u32* lb = lds + r * RS;
-    u32* la = lds + (rpb + r) * RS;
 
     if constexpr (MODE == ROW_FUSED) {
         const u32* ra = RA + off;
@@ -260,22 +259,22 @@ __device__ __forceinline__ void row_tile(const DevPlan& D, u32* __restrict__ RB,
             fwd_pass1<NB, 2>(x, tab, s0, row, p);
         }
         if constexpr (R2 > 0) {
-            if (on) {
+            // a, then b, through the one row buffer: half the LDS of a
+            // buffer each, so twice the rows (and waves) fit on a CU
 #pragma unroll
-                for (int k = 0; k < E; ++k) {
-                    la[swz(t + k * T)] = x[0][k];
-                    lb[swz(t + k * T)] = x[1][k];
+            for (int a = 0; a < 2; ++a) {
+                if (a) __syncthreads();  // every read of a's transpose is done
+                if (on) {
+#pragma unroll
+                    for (int k = 0; k < E; ++k) lb[swz(t + k * T)] = x[a][k];
+                }
+                __syncthreads();
+                if (on) {
+#pragma unroll
+                    for (int k = 0; k < E; ++k) x[a][k] = lb[swz(t * E + k)];
                 }
             }
-            __syncthreads();
-            if (on) {
-#pragma unroll
-                for (int k = 0; k < E; ++k) {
-                    x[0][k] = la[swz(t * E + k)];
-                    x[1][k] = lb[swz(t * E + k)];
-                }
-                fwd_pass2<NB, 2>(x, tab, s0, row, t, p);
-            }
+            if (on) fwd_pass2<NB, 2>(x, tab, s0, row, t, p);
         }
         u32 y[E];
         if (on) {
@@ -351,8 +350,11 @@ __device__ __forceinline__ void row_tile(const DevPlan& D, u32* __restrict__ RB,
     }
 }
 
+// At most 80 VGPRs (6 waves per SIMD): with one row buffer per row the LDS
+// allows that many resident blocks; 64 would spill.
 template <int NB, int MODE>
-__global__ void __launch_bounds__(256) k_row(DevPlan D, u32* __restrict__ RB, const u32* __restrict__ RA, int rpb) {
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(6, 8)))
+k_row(DevPlan D, u32* __restrict__ RB, const u32* __restrict__ RA, int rpb) {
     extern __shared__ u32 lds[];
     row_tile<NB, MODE>(D, RB, RA, (int)blockIdx.y, blockIdx.x * (u32)rpb, rpb, lds, (int)threadIdx.x);
 }
@@ -804,19 +806,19 @@ int line_threads(int nb) { return nb >= 4 ? 1 << (nb - 4) : 1; }
 
 }  // namespace
 
-// PM_RNS_TN / PM_CRT_TN (32, 64 or 128) cap the coefficients per RNS / CRT
-// block, for tuning; the LDS limit still applies.
-static int tile_cap(const char* name) {
+// PM_RNS_TN (32 .. 256) / PM_CRT_TN (32 .. 128) set the coefficients per
+// RNS / CRT block, for tuning; the LDS limit still applies.
+static int tile_cap(const char* name, int dflt, int maxv) {
     const char* v = std::getenv(name);
-    const int t = v ? std::atoi(v) : 128;
-    return (t == 32 || t == 64 || t == 128) ? t : 128;
+    const int t = v ? std::atoi(v) : dflt;
+    return (t == 32 || t == 64 || t == 128 || t == 256) && t <= maxv ? t : dflt;
 }
 
 LaunchCfg choose_launch(const DevPlan& D, size_t max_lds) {
     LaunchCfg C;
     const u32 nrows = 1u << D.w1, ncols = 1u << D.w2;
     C.rns_stride = 4 * D.L;
-    C.rns_tn = tile_cap("PM_RNS_TN");
+    C.rns_tn = tile_cap("PM_RNS_TN", 128, 256);
     while (C.rns_tn > 32 && (size_t)C.rns_tn * C.rns_stride + 32 > max_lds) C.rns_tn /= 2;
     C.rns_lds = (size_t)C.rns_tn * C.rns_stride + 32;
     // 4-wave RNS blocks by default: four blocks per CU overlap their tile
@@ -826,7 +828,7 @@ LaunchCfg choose_launch(const DevPlan& D, size_t max_lds) {
     // 32 coefficients per CRT block by default: half the LDS of 64, so more
     // blocks per CU hide the latency of the residue loads (measured faster at
     // N=65536).
-    C.crt_tn = std::getenv("PM_CRT_TN") ? tile_cap("PM_CRT_TN") : 32;
+    C.crt_tn = tile_cap("PM_CRT_TN", 32, 128);
     auto crt_need = [&](int tn) { return (size_t)tn * C.crt_ys + (size_t)tn * (D.L + 3) * 8; };
     while (C.crt_tn > 32 && crt_need(C.crt_tn) > max_lds) C.crt_tn /= 2;
     C.crt_lds = crt_need(C.crt_tn);
@@ -849,6 +851,7 @@ hipError_t launch_rns(const DevPlan& D, const LaunchCfg& C, const u32* a0, const
     }
     const int vec = ((uintptr_t)a0 % 16 == 0) && (nops < 2 || (uintptr_t)a1 % 16 == 0);
     switch (C.rns_tn) {
+        case 256: hipLaunchKernelGGL((k_rns<8>), grid, block, C.rns_lds, s, D, a0, a1, R0, R1, count, vec); break;
         case 128: hipLaunchKernelGGL((k_rns<4>), grid, block, C.rns_lds, s, D, a0, a1, R0, R1, count, vec); break;
         case 64: hipLaunchKernelGGL((k_rns<2>), grid, block, C.rns_lds, s, D, a0, a1, R0, R1, count, vec); break;
         case 32: hipLaunchKernelGGL((k_rns<1>), grid, block, C.rns_lds, s, D, a0, a1, R0, R1, count, vec); break;
@@ -872,7 +875,7 @@ hipError_t launch_row(const DevPlan& D, const LaunchCfg& C, u32* RB, const u32* 
     const int T = line_threads(D.w2);
     const dim3 grid(nrows / C.row_rpb, D.K), block(C.row_rpb * T);
     const size_t rs = ((size_t)1 << D.w2) + (D.w2 > 4 ? 16 : 0);
-    const size_t lds = D.w2 > 4 ? 2 * (size_t)C.row_rpb * rs * 4 : 0;
+    const size_t lds = D.w2 > 4 ? (size_t)C.row_rpb * rs * 4 : 0;
     switch (mode) {
         case ROW_FUSED: return row_dispatch<ROW_FUSED>(D.w2, grid, block, lds, s, D, RB, RA, C.row_rpb);
         case ROW_FWD: return row_dispatch<ROW_FWD>(D.w2, grid, block, lds, s, D, RB, RA, C.row_rpb);
