@@ -330,6 +330,9 @@ struct Batch {
         graph = nullptr;
     }
     void configure(const BatchConfig& c) {
+        // launches still in flight (on any stream) read the buffers the new
+        // plan may reallocate
+        hip_check(hipDeviceSynchronize(), "synchronize");
         run.plan(c);
         run.upload(stream);
         hip_check(hipStreamSynchronize(stream), "upload");
@@ -349,10 +352,17 @@ struct Batch {
                 changed = true;
             }
         }
-        if (changed)
-            hip_check(hipMemcpyAsync(run.d_prods, run.prods.data(), run.prods.size() * sizeof(BatchProduct),
-                                     hipMemcpyHostToDevice, stream),
-                      "upload products");
+        if (!changed) return;
+        hip_check(hipMemcpyAsync(run.d_prods, run.prods.data(), run.prods.size() * sizeof(BatchProduct),
+                                 hipMemcpyHostToDevice, stream),
+                  "upload products");
+        // The phase kernels read the pointers from d_prods, but a group of one
+        // product was recorded with them as kernel arguments: capture again.
+        for (int g : run.single)
+            if (g >= 0) {
+                drop_graph();
+                break;
+            }
     }
     void enqueue(hipStream_t st) {
         if (!exec) {

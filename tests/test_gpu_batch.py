@@ -105,3 +105,26 @@ def test_batch_configurations(O, pm, cfg):
         B.run(da, db, dc)
         B.sync()
         _check(O, prods, ops, dc)
+
+
+def test_batch_new_buffers_each_launch(O, pm):
+    """A launch with other operand and result buffers computes on those (the
+    graph records single-product groups with their buffers as arguments)."""
+    points = [(65536, 1747), (4096, 256), (2048, 868)]
+    B, prods, ops, da, db, dc = _batch(O, pm, points, 600, "uniform")
+    B.configure(col_tile=32, ceil_split=0, blocks_per_cu=0, group_mb=96, branches=2)
+    B.run(da, db, dc)
+    B.sync()
+    _check(O, prods, ops, dc)
+    ops2 = [O.operands("mixed", N, W, q, 700 + i) for i, (N, W, q) in enumerate(prods)]
+    da2 = [pm.DeviceBuffer.from_array(a) for a, _ in ops2]
+    db2 = [pm.DeviceBuffer.from_array(b) for _, b in ops2]
+    dc2 = [pm.DeviceBuffer(N * O.limbs_of(W) * 4) for N, W in points]
+    B.run(da2, db2, dc2)
+    B.sync()
+    _check(O, prods, ops2, dc2, idx=[1, 2])
+    N, W, q = prods[0]
+    with pm.Multiplier(N, W, O.q_limbs(q, O.limbs_of(W))) as m:
+        assert np.array_equal(dc2[0].download((N, O.limbs_of(W))), m.run(*ops2[0]))
+    # and the first buffers were left alone by the second launch
+    _check(O, prods, ops, dc, idx=[1, 2])
