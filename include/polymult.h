@@ -47,10 +47,14 @@ int pm_init(uint32_t N, uint32_t W, uint32_t L, const uint32_t* q, size_t q_limb
  * blocking. Replaces fherma_run (fherma.h:47; solve.cu:1480-1529). */
 int pm_run(pm_state* s, const uint32_t* a, const uint32_t* b, uint32_t* c);
 
-/* c = a * b with a, b, c already in device memory (N*L u32 each), enqueued
- * on `stream` (a hipStream_t; NULL = the state's own stream). Returns once
- * enqueued. No counterpart in the reference: this is the device-resident
- * form of the same call. */
+/* c = a * b with a, b, c already in device memory (N*L u32 each). Returns
+ * once enqueued. With a `stream` (a hipStream_t) the call runs in order on
+ * it. With NULL, consecutive calls alternate over the state's two internal
+ * lanes (each with its own residue buffers), so one call's kernels overlap
+ * the next call's; results still land in call order (a call's final kernel
+ * waits for the previous call's), and pm_sync waits for both lanes. Inputs
+ * must not change before pm_sync. No counterpart in the reference: this is
+ * the device-resident form of the same call. */
 int pm_run_device(pm_state* s, const uint32_t* d_a, const uint32_t* d_b, uint32_t* d_c, void* stream);
 
 /* Releases everything. Replaces fherma_free (fherma.h:48; solve.cu:1531-1535). */
@@ -69,6 +73,7 @@ int pm_dev_alloc(size_t bytes, void** out);
 int pm_dev_free(void* p);
 int pm_copy_to_device(void* dst, const void* src, size_t bytes);
 int pm_copy_to_host(void* dst, const void* src, size_t bytes);
+/* Waits for every device call of the state (both lanes). */
 int pm_sync(pm_state* s);
 
 /* Runs `iters` device-resident multiplies back to back on the state's stream

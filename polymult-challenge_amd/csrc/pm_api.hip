@@ -128,12 +128,45 @@ void State::init(uint32_t N, uint32_t W, uint32_t L, const uint32_t* q, size_t q
 }
 
 hipError_t State::enqueue_multiply(const uint32_t* a, const uint32_t* b, uint32_t* c, hipStream_t st) {
+    return enqueue_multiply_with(a, b, c, st, d_R[0], d_R[1]);
+}
+
+hipError_t State::enqueue_multiply_with(const uint32_t* a, const uint32_t* b, uint32_t* c, hipStream_t st,
+                                        uint32_t* R0, uint32_t* R1) {
     hipError_t e;
-    if ((e = launch_rns(dev, cfg, a, b, d_R[0], d_R[1], 2, st)) != hipSuccess) return e;
-    if ((e = launch_col(dev, cfg, d_R[0], d_R[1], 2, false, st)) != hipSuccess) return e;
-    if ((e = launch_row(dev, cfg, d_R[1], d_R[0], ROW_FUSED, st)) != hipSuccess) return e;
-    if ((e = launch_col(dev, cfg, d_R[1], nullptr, 1, true, st)) != hipSuccess) return e;
-    return launch_crt(dev, cfg, d_R[1], c, dev.G, st);
+    if ((e = launch_rns(dev, cfg, a, b, R0, R1, 2, st)) != hipSuccess) return e;
+    if ((e = launch_col(dev, cfg, R0, R1, 2, false, st)) != hipSuccess) return e;
+    if ((e = launch_row(dev, cfg, R1, R0, ROW_FUSED, st)) != hipSuccess) return e;
+    if ((e = launch_col(dev, cfg, R1, nullptr, 1, true, st)) != hipSuccess) return e;
+    return launch_crt(dev, cfg, R1, c, dev.G, st);
+}
+
+void State::enqueue_lanes(const uint32_t* a, const uint32_t* b, uint32_t* c) {
+    if (!lane2) {
+        const size_t rbytes = (size_t)host.K * host.N * 4;
+        for (int i = 0; i < 2; ++i) hip_check(hipMalloc(&d_R2[i], rbytes), "hipMalloc residues");
+        for (int i = 0; i < 2; ++i) hip_check(hipEventCreateWithFlags(&ev_crt[i], hipEventDisableTiming), "event");
+        hip_check(hipStreamCreateWithFlags(&lane2, hipStreamNonBlocking), "stream");
+        device_bytes += 2 * rbytes;
+    }
+    const int L = next_lane;
+    next_lane ^= 1;
+    hipStream_t st = L ? lane2 : stream;
+    uint32_t* R0 = L ? d_R2[0] : d_R[0];
+    uint32_t* R1 = L ? d_R2[1] : d_R[1];
+    hip_check(launch_rns(dev, cfg, a, b, R0, R1, 2, st), "rns");
+    hip_check(launch_col(dev, cfg, R0, R1, 2, false, st), "col");
+    hip_check(launch_row(dev, cfg, R1, R0, ROW_FUSED, st), "row");
+    hip_check(launch_col(dev, cfg, R1, nullptr, 1, true, st), "col inv");
+    if (crt_recorded[1 - L]) hip_check(hipStreamWaitEvent(st, ev_crt[1 - L], 0), "wait");
+    hip_check(launch_crt(dev, cfg, R1, c, dev.G, st), "crt");
+    hip_check(hipEventRecord(ev_crt[L], st), "record");
+    crt_recorded[L] = true;
+}
+
+void State::sync_all() {
+    hip_check(hipStreamSynchronize(stream), "synchronize");
+    if (lane2) hip_check(hipStreamSynchronize(lane2), "synchronize");
 }
 
 std::vector<uint32_t> State::shard_bounds(int shards) const {
@@ -203,6 +236,12 @@ void State::enqueue_sharded(const uint32_t* a, const uint32_t* b, uint32_t* c, i
 
 State::~State() {
     if (stream) (void)hipStreamSynchronize(stream);
+    if (lane2) (void)hipStreamSynchronize(lane2);
+    for (auto& e : ev_crt)
+        if (e) (void)hipEventDestroy(e);
+    for (auto& r : d_R2)
+        if (r) (void)hipFree(r);
+    if (lane2) (void)hipStreamDestroy(lane2);
     if (d_part) (void)hipFree(d_part);
     pipeline_free();
     for (int i = 0; i < 2; ++i) {
@@ -274,13 +313,15 @@ int pm_run(pm_state* s, const uint32_t* a, const uint32_t* b, uint32_t* c) {
 int pm_run_device(pm_state* s, const uint32_t* d_a, const uint32_t* d_b, uint32_t* d_c, void* stream) {
     return guarded([&] {
         if (!s || !d_a || !d_b || !d_c) throw std::runtime_error("null argument");
-        hipStream_t st = stream ? static_cast<hipStream_t>(stream) : s->s->stream;
-        hip_check(s->s->enqueue_multiply(d_a, d_b, d_c, st), "multiply");
+        if (stream)
+            hip_check(s->s->enqueue_multiply(d_a, d_b, d_c, static_cast<hipStream_t>(stream)), "multiply");
+        else
+            s->s->enqueue_lanes(d_a, d_b, d_c);
     });
 }
 
 int pm_sync(pm_state* s) {
-    return guarded([&] { hip_check(hipStreamSynchronize(s->s->stream), "synchronize"); });
+    return guarded([&] { s->s->sync_all(); });
 }
 
 int pm_dev_alloc(size_t bytes, void** out) {

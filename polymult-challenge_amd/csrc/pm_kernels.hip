@@ -409,22 +409,27 @@ __device__ __forceinline__ void rns_tile(const DevPlan& D, const u32* __restrict
     const u32* src = a + (size_t)n0 * L;
     const int words = TN * L;
 
+    // The bytes go into LDS already re-centred (^0x80 per byte), once, rather
+    // than at every fragment load (each is read by every row tile).
+    constexpr u32 kC = 0x80808080u;
     if (vec && cnt == TN) {
         const uint4* s4 = reinterpret_cast<const uint4*>(src);
         uint4* d4 = reinterpret_cast<uint4*>(lds32);
-        for (int i = tid; i < words / 4; i += nthreads) d4[i] = s4[i];
+        for (int i = tid; i < words / 4; i += nthreads) {
+            const uint4 v = s4[i];
+            d4[i] = make_uint4(v.x ^ kC, v.y ^ kC, v.z ^ kC, v.w ^ kC);
+        }
     } else {
-        for (int i = tid; i < words; i += nthreads) lds32[i] = i < cnt * L ? src[i] : 0u;
+        for (int i = tid; i < words; i += nthreads) lds32[i] = (i < cnt * L ? src[i] : 0u) ^ kC;
     }
-    if (tid < 8) lds32[words + tid] = 0x80808080u;
+    if (tid < 8) lds32[words + tid] = 0u;  // read by the last k-step of the last column; digits there are zero
     __syncthreads();
 
     const int lane = tid & 63, wave = tid >> 6, nw = nthreads >> 6;
     const int half = lane >> 5, col = lane & 31;
     auto bfrag = [&](int ct, int ks) {
         const u32* q = lds32 + (32 * ct + col) * L + 8 * ks + 4 * half;
-        const v4i B = {(int)(q[0] ^ 0x80808080u), (int)(q[1] ^ 0x80808080u), (int)(q[2] ^ 0x80808080u),
-                       (int)(q[3] ^ 0x80808080u)};
+        const v4i B = {(int)q[0], (int)q[1], (int)q[2], (int)q[3]};
         return B;
     };
     for (int rt = wave; rt < D.rns_rt; rt += nw) {

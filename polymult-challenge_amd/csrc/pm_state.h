@@ -35,6 +35,20 @@ struct State {
     void upload_plan();
     // The whole multiply on device buffers, enqueued on st.
     hipError_t enqueue_multiply(const uint32_t* a, const uint32_t* b, uint32_t* c, hipStream_t st);
+    hipError_t enqueue_multiply_with(const uint32_t* a, const uint32_t* b, uint32_t* c, hipStream_t st,
+                                     uint32_t* R0, uint32_t* R1);
+
+    // Device calls without a caller stream alternate over two lanes (the
+    // state's stream and lane2, each with its own residues), so one call's
+    // latency-bound kernels overlap the next call's. A call's CRT waits for
+    // the previous call's CRT, so results land in call order.
+    hipStream_t lane2 = nullptr;
+    uint32_t* d_R2[2] = {nullptr, nullptr};
+    hipEvent_t ev_crt[2] = {nullptr, nullptr};
+    bool crt_recorded[2] = {false, false};
+    int next_lane = 0;
+    void enqueue_lanes(const uint32_t* a, const uint32_t* b, uint32_t* c);
+    void sync_all();
 
     // Prime shards (the multi-GPU split, run here one shard after another):
     // boundaries j_0 = 0 < j_1 < ... < j_G = K on whole MFMA k-steps (8

@@ -158,3 +158,28 @@ def test_errors_like_reference(O, pm):
     m = pm.Multiplier(64, 32, O.q_limbs(q, 1))
     with pytest.raises(pm.PolymultError, match="operand size"):
         m.run(np.zeros(10, np.uint32), np.zeros(10, np.uint32))
+
+
+def test_device_calls_in_flight(O, pm):
+    """Default-stream device calls overlap on two lanes: each result is exact,
+    and calls into one output buffer leave the last call's result."""
+    N, W = 32768, 868
+    q = O.spec_q(N, W)
+    L = O.limbs_of(W)
+    ops = [O.operands(k, N, W, q, 900 + i) for i, k in enumerate(["uniform", "mixed", "max", "sparse", "small16"])]
+    with pm.Multiplier(N, W, O.q_limbs(q, L)) as m:
+        da = [pm.DeviceBuffer.from_array(a) for a, _ in ops]
+        db = [pm.DeviceBuffer.from_array(b) for _, b in ops]
+        dc = [pm.DeviceBuffer(N * L * 4) for _ in ops]
+        for rep in range(3):
+            for i in range(len(ops)):
+                m.run_device(da[i], db[i], dc[i])
+        shared = pm.DeviceBuffer(N * L * 4)
+        for i in range(len(ops)):
+            m.run_device(da[i], db[i], shared)
+        m.sync()
+        for i, (a, b) in enumerate(ops):
+            assert np.array_equal(dc[i].download((N, L)), O.OraclePlan(N, W, q).multiply(a, b)), i
+        assert np.array_equal(shared.download((N, L)), dc[-1].download((N, L)))
+        # the host-buffer path on the same state afterwards
+        assert np.array_equal(m.run(*ops[1]), dc[1].download((N, L)))
