@@ -148,12 +148,13 @@ int pm_batch_time(pm_batch* b, const uint32_t* const* d_a, const uint32_t* const
  * logN: the larger half on the column pass), blocks_per_cu 0 (one block per
  * tile) or 1..8 (persistent blocks claiming tiles), group_mb (products run in
  * groups of at most that many MiB of residues; 0: one group; a group of one
- * product runs the single-product kernels), branches 1/2 (graph chains). */
+ * product runs the single-product kernels), branches 1..3 (graph chains the
+ * groups go round-robin over). */
 int pm_batch_configure(pm_batch* b, int col_tile, int ceil_split, int blocks_per_cu, int group_mb, int branches);
 /* Times every candidate (column tile 8/16/32, 4-step split of odd logN,
  * tiles handed out by the dispatcher (0) or by 2 persistent blocks per CU,
- * product groups of 48/96/192 MiB of residues or one group, one or two graph
- * branches), keeps the fastest; best[0..5] = column tile, split, blocks per
+ * product groups of 48/96/192 MiB of residues or one group, one to three
+ * graph branches), keeps the fastest; best[0..5] = column tile, split, blocks per
  * CU, microseconds per launch, group MiB, branches. */
 int pm_batch_autotune(pm_batch* b, const uint32_t* const* d_a, const uint32_t* const* d_b, uint32_t* const* d_c,
                       int iters, int* best);

@@ -309,8 +309,9 @@ struct Batch {
     Runner run;
     uint32_t* d_R = nullptr;
     hipStream_t stream = nullptr;
-    hipStream_t stream2 = nullptr;  // second branch, used only inside the capture
-    hipEvent_t fork = nullptr, join = nullptr;
+    static constexpr int kMaxBranches = 3;
+    hipStream_t side[kMaxBranches - 1] = {};  // further branches, used only inside the capture
+    hipEvent_t fork = nullptr, join[kMaxBranches - 1] = {};
     hipGraph_t graph = nullptr;
     hipGraphExec_t exec = nullptr;
 
@@ -319,8 +320,10 @@ struct Batch {
         drop_graph();
         if (d_R) (void)hipFree(d_R);
         if (fork) (void)hipEventDestroy(fork);
-        if (join) (void)hipEventDestroy(join);
-        if (stream2) (void)hipStreamDestroy(stream2);
+        for (auto e : join)
+            if (e) (void)hipEventDestroy(e);
+        for (auto t : side)
+            if (t) (void)hipStreamDestroy(t);
         if (stream) (void)hipStreamDestroy(stream);
     }
     void drop_graph() {
@@ -368,14 +371,17 @@ struct Batch {
         if (!exec) {
             hip_check(hipStreamBeginCapture(stream, hipStreamCaptureModeThreadLocal), "capture");
             hipError_t e = run.launch_reset(stream);
-            if (run.cfg.branches > 1) {
-                // groups alternate over two chains that fork and join inside the graph
+            const int nb = std::max(1, std::min(run.cfg.branches, kMaxBranches));
+            if (nb > 1) {
+                // groups go round-robin over nb chains that fork and join inside the graph
                 if (e == hipSuccess) e = hipEventRecord(fork, stream);
-                if (e == hipSuccess) e = hipStreamWaitEvent(stream2, fork, 0);
+                for (int k = 0; e == hipSuccess && k < nb - 1; ++k) e = hipStreamWaitEvent(side[k], fork, 0);
                 for (size_t g = 0; e == hipSuccess && g < run.groups(); ++g)
-                    e = run.launch_group(g, g % 2 ? stream2 : stream);
-                if (e == hipSuccess) e = hipEventRecord(join, stream2);
-                if (e == hipSuccess) e = hipStreamWaitEvent(stream, join, 0);
+                    e = run.launch_group(g, g % nb ? side[g % nb - 1] : stream);
+                for (int k = 0; e == hipSuccess && k < nb - 1; ++k) {
+                    e = hipEventRecord(join[k], side[k]);
+                    if (e == hipSuccess) e = hipStreamWaitEvent(stream, join[k], 0);
+                }
             } else {
                 for (size_t g = 0; e == hipSuccess && g < run.groups(); ++g) e = run.launch_group(g, stream);
             }
@@ -410,9 +416,9 @@ void batch_build(Batch& B, uint32_t count, const uint32_t* N, const uint32_t* W,
     B.cat.build(count, N, W, q_concat);
     B.run.init();
     hip_check(hipStreamCreateWithFlags(&B.stream, hipStreamNonBlocking), "stream");
-    hip_check(hipStreamCreateWithFlags(&B.stream2, hipStreamNonBlocking), "stream");
+    for (auto& t : B.side) hip_check(hipStreamCreateWithFlags(&t, hipStreamNonBlocking), "stream");
     hip_check(hipEventCreateWithFlags(&B.fork, hipEventDisableTiming), "event");
-    hip_check(hipEventCreateWithFlags(&B.join, hipEventDisableTiming), "event");
+    for (auto& e : B.join) hip_check(hipEventCreateWithFlags(&e, hipEventDisableTiming), "event");
     size_t rwords = 0;
     for (size_t i = 0; i < count; ++i) rwords += B.cat.residue_words(i);
     hip_check(hipMalloc(&B.d_R, rwords * 4), "hipMalloc residues");
@@ -524,7 +530,7 @@ int pm_batch_configure(pm_batch* b, int col_tile, int ceil_split, int blocks_per
         if (!b) throw std::runtime_error("null argument");
         if (col_tile != 8 && col_tile != 16 && col_tile != 32) throw std::runtime_error("col_tile must be 8, 16 or 32");
         if (ceil_split < 0 || ceil_split > 1 || blocks_per_cu < 0 || blocks_per_cu > 8 || group_mb < 0 ||
-            branches < 1 || branches > 2)
+            branches < 1 || branches > pm::Batch::kMaxBranches)
             throw std::runtime_error("batch configuration out of range");
         pm::BatchConfig c;
         c.col_tc = col_tile;
@@ -547,7 +553,7 @@ int pm_batch_autotune(pm_batch* b, const uint32_t* const* d_a, const uint32_t* c
             for (int split = 0; split < 2; ++split)
                 for (int bpc : {0, 2})
                     for (int gmb : {0, 48, 96, 192})
-                        for (int br : {1, 2}) {
+                        for (int br : {1, 2, 3}) {
                     pm::BatchConfig c;
                     c.col_tc = tc;
                     c.ceil_split = split;

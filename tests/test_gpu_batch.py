@@ -91,6 +91,7 @@ def test_batch_closed_form_full_size(O, pm):
     dict(col_tile=16, ceil_split=0, blocks_per_cu=0, group_mb=1, branches=2),
     dict(col_tile=32, ceil_split=1, blocks_per_cu=2, group_mb=4, branches=2),
     dict(col_tile=32, ceil_split=0, blocks_per_cu=0, group_mb=0, branches=2),
+    dict(col_tile=32, ceil_split=0, blocks_per_cu=2, group_mb=1, branches=3),
 ])
 def test_batch_configurations(O, pm, cfg):
     """Every hand-out mode, grouping (including groups of one product, which
