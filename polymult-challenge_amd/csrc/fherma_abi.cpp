@@ -6,6 +6,7 @@
 // the next result vector between calls; the call only takes it. The helper
 // never sees operand data.
 #include <condition_variable>
+#include <memory>
 #include <mutex>
 #include <thread>
 
@@ -65,7 +66,11 @@ class NextResult {
 struct FhermaState {
     pm_state* s = nullptr;
     uint32_t N = 0, L = 0;
-    NextResult* next = nullptr;
+    std::unique_ptr<NextResult> next;
+    ~FhermaState() {
+        next.reset();  // stop the helper before the state goes
+        pm_free(s);
+    }
 };
 
 }  // namespace
@@ -74,12 +79,12 @@ void* fherma_init(const fherma::Point& pt) {
     pm_state* s = nullptr;
     if (pm_init(pt.N, pt.W, pt.L, pt.q.data.data(), pt.q.data.size(), &s) != 0)
         throw std::runtime_error(pm_last_error());
-    auto* F = new FhermaState;
+    auto F = std::make_unique<FhermaState>();  // owns s from here on
     F->s = s;
     F->N = pt.N;
     F->L = pt.L;
-    F->next = new NextResult((size_t)pt.N * pt.L);
-    return F;
+    F->next = std::make_unique<NextResult>((size_t)pt.N * pt.L);
+    return F.release();
 }
 
 fherma::Outputs fherma_run(void* state, const fherma::Inputs& in) {
@@ -96,10 +101,4 @@ fherma::Outputs fherma_run(void* state, const fherma::Inputs& in) {
     return out;
 }
 
-void fherma_free(void* state) {
-    auto* F = static_cast<FhermaState*>(state);
-    if (!F) return;
-    delete F->next;
-    pm_free(F->s);
-    delete F;
-}
+void fherma_free(void* state) { delete static_cast<FhermaState*>(state); }

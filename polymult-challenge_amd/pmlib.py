@@ -216,6 +216,7 @@ class DeviceBuffer:
         _check(load().pm_copy_to_device(self.ptr, a.ctypes.data_as(_vp), a.nbytes))
 
     def download(self, shape, dtype=np.uint32) -> np.ndarray:
+        """Synchronous copy to host; sync() the work that writes this buffer first."""
         out = np.empty(shape, dtype=dtype)
         _check(load().pm_copy_to_host(out.ctypes.data_as(_vp), self.ptr, out.nbytes))
         return out
@@ -272,9 +273,13 @@ class Multiplier:
         return c
 
     def run_device(self, da: DeviceBuffer, db: DeviceBuffer, dc: DeviceBuffer, stream=None):
+        """Enqueues c = a b on device buffers and returns. Without a stream,
+        consecutive calls overlap on two internal lanes (results still land
+        in call order); call sync() before reading dc or changing da, db."""
         _check(load().pm_run_device(self._h, da.ptr, db.ptr, dc.ptr, stream))
 
     def sync(self):
+        """Waits for every device call of this multiplier."""
         _check(load().pm_sync(self._h))
 
     def time_device(self, da: DeviceBuffer, db: DeviceBuffer, dc: DeviceBuffer, iters: int) -> list[float]:
