@@ -197,7 +197,7 @@ def main():
                     help="only run the plain device copy kernel over BYTES (the counter calibration) and exit")
     args = ap.parse_args()
     if args.gpus != 1:
-        print("bench.py measures one GPU: this build runs on a single MI355X (the path would shard by RNS "
+        print("bench.py measures one GPU: the multiply runs on a single MI355X (the path would shard by RNS "
               "prime across GPUs, see the design notes); --gpus %d is not supported" % args.gpus, file=sys.stderr)
         sys.exit(2)
     legs = set(args.legs.split(","))

@@ -7,8 +7,9 @@ bench.py's CPU-baseline leg import this module; the product never does.
 What is computed here, and what pins it:
   spec_q          the spec's modulus: the largest prime q < 2^W with
                   q = 1 (mod 2N) (reference tests/oracle.py:48-57 restated).
-                  Pinned by the delta = 2^W - q values the survey derived with
-                  sympy, and by q = 1 (mod 2N) / primality checks.
+                  Pinned by delta = 2^W - q values derived independently with
+                  sympy (tests/test_oracle.py), and by q = 1 (mod 2N) /
+                  primality checks.
   schoolbook      negacyclic product by definition (tests/oracle.py:131-140
                   restated); the ground truth for N <= 256.
   kronecker       an NTT-free bigint product: pack, one big multiply, fold

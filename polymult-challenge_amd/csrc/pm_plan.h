@@ -9,7 +9,7 @@
 
 namespace pm {
 
-// Limits of this build. N above 2^16 would need a third NTT pass.
+// Limits: N above 2^16 would need a third NTT pass.
 constexpr int kMaxLogN = 16;
 constexpr int kMaxK = 1023;
 
