@@ -106,3 +106,33 @@ def test_serve_saturate_closed_loop(O, pm):
         print("saturate", {k: rep[k] for k in ("requests", "completed", "achieved_rate", "mean_batch")}, flush=True)
         assert rep["completed"] == rep["requests"] > 16 and rep["failed"] == 0
         assert rep["mean_batch"] > 1.0
+
+
+def test_serve_concurrent_submitters(O, pm):
+    """Four threads submit and wait at once (the binding releases the GIL in
+    every call); every result is exact."""
+    import threading
+    cat = _catalog(O, [(4096, 256), (8192, 438)])
+    with pm.Server(cat) as S:
+        errors = []
+
+        def worker(t):
+            try:
+                for i in range(6):
+                    p = (t + i) % 2
+                    N, W, q = cat[p]
+                    a, b = O.operands("uniform", N, W, q, 1000 + 10 * t + i)
+                    c, _ = S.wait(S.submit(p, a, b))
+                    if not np.array_equal(c, O.OraclePlan(N, W, q).multiply(a, b)):
+                        errors.append((t, i))
+            except Exception as e:  # surfaced below
+                errors.append(repr(e))
+
+        threads = [threading.Thread(target=worker, args=(t,)) for t in range(4)]
+        for th in threads:
+            th.start()
+        for th in threads:
+            th.join(120)
+        assert not any(th.is_alive() for th in threads)
+        assert not errors, errors
+        assert S.stats()["completed"] == 24
