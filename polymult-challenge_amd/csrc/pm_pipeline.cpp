@@ -6,7 +6,8 @@
 //     copies chunk i from the caller's pageable buffer into pinned staging
 //     (non-temporal stores) while the copy engine moves chunk i-1 to HBM and
 //     the RNS kernel decomposes chunk i-2 on the compute stream;
-//   * once every chunk is resident the three NTT passes run;
+//   * a's forward column pass runs once a's chunks are in (while b's are
+//     still arriving); once every chunk is resident the other passes run;
 //   * the result leaves in chunks: CRT of chunk i on the compute stream, its
 //     device-to-host copy on a second copy stream, and the host team scatters
 //     chunk i-1 from pinned staging into the caller's buffer.
@@ -282,6 +283,8 @@ void State::run_host(const uint32_t* a, const uint32_t* b, uint32_t* c) {
             hip_check(hipStreamWaitEvent(stream, P.ev_in[i], 0), "wait");
             hip_check(launch_rns(dev, cfg, d_in[op], nullptr, d_R[op], nullptr, 1, stream, ch.first, ch.count), "rns");
         }
+        // a's column pass runs while b's chunks are still crossing the link
+        if (op == 0) hip_check(launch_col(dev, cfg, d_R[0], nullptr, 1, false, stream), "col a");
     }
     const auto t_in = clock::now();
     if (P.profile) {
@@ -290,8 +293,8 @@ void State::run_host(const uint32_t* a, const uint32_t* b, uint32_t* c) {
         hip_check(hipEventRecord(P.tl[2], stream), "record");
     }
 
-    // ---- the three NTT passes ----
-    hip_check(launch_col(dev, cfg, d_R[0], d_R[1], 2, false, stream), "col");
+    // ---- the rest of the NTT passes ----
+    hip_check(launch_col(dev, cfg, d_R[1], nullptr, 1, false, stream), "col b");
     hip_check(launch_row(dev, cfg, d_R[1], d_R[0], ROW_FUSED, stream), "row");
     hip_check(launch_col(dev, cfg, d_R[1], nullptr, 1, true, stream), "col inv");
     if (P.profile) hip_check(hipEventRecord(P.tl_more[0], stream), "record");
