@@ -49,10 +49,11 @@ int pm_run(pm_state* s, const uint32_t* a, const uint32_t* b, uint32_t* c);
 
 /* c = a * b with a, b, c already in device memory (N*L u32 each). Returns
  * once enqueued. With a `stream` (a hipStream_t) the call runs in order on
- * it. With NULL, consecutive calls alternate over the state's two internal
- * lanes (each with its own residue buffers), so one call's kernels overlap
- * the next call's; results still land in call order (a call's final kernel
- * waits for the previous call's), and pm_sync waits for both lanes. Inputs
+ * it. With NULL, consecutive calls rotate over the state's internal lanes
+ * (two by default, PM_LANES=1..4; each with its own residue buffers), so one
+ * call's kernels overlap the next call's; results still land in call order
+ * (a call's final kernel waits for the previous call's), and pm_sync waits
+ * for every lane. Inputs
  * must not change before pm_sync. No counterpart in the reference: this is
  * the device-resident form of the same call. */
 int pm_run_device(pm_state* s, const uint32_t* d_a, const uint32_t* d_b, uint32_t* d_c, void* stream);
@@ -73,7 +74,7 @@ int pm_dev_alloc(size_t bytes, void** out);
 int pm_dev_free(void* p);
 int pm_copy_to_device(void* dst, const void* src, size_t bytes);
 int pm_copy_to_host(void* dst, const void* src, size_t bytes);
-/* Waits for every device call of the state (both lanes). */
+/* Waits for every device call of the state (every lane). */
 int pm_sync(pm_state* s);
 
 /* Runs `iters` device-resident multiplies back to back on the state's stream

@@ -4,6 +4,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <cstdlib>
 #include <cstring>
 #include <memory>
 #include <stdexcept>
@@ -107,6 +108,7 @@ void State::init(uint32_t N, uint32_t W, uint32_t L, const uint32_t* q, size_t q
     int max_lds = 0;
     hip_check(hipDeviceGetAttribute(&max_lds, hipDeviceAttributeMaxSharedMemoryPerBlock, device), "device attribute");
     hip_check(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking), "stream");
+    if (const char* e = std::getenv("PM_LANES")) lanes = std::min(kMaxLanes, std::max(1, std::atoi(e)));
     upload_plan();
     cfg = choose_launch(dev, std::min<size_t>((size_t)max_lds, 64 * 1024));
     if (cfg.crt_lds > (size_t)max_lds || cfg.rns_lds > (size_t)max_lds)
@@ -142,31 +144,38 @@ hipError_t State::enqueue_multiply_with(const uint32_t* a, const uint32_t* b, ui
 }
 
 void State::enqueue_lanes(const uint32_t* a, const uint32_t* b, uint32_t* c) {
-    if (!lane2) {
-        const size_t rbytes = (size_t)host.K * host.N * 4;
-        for (int i = 0; i < 2; ++i) hip_check(hipMalloc(&d_R2[i], rbytes), "hipMalloc residues");
-        for (int i = 0; i < 2; ++i) hip_check(hipEventCreateWithFlags(&ev_crt[i], hipEventDisableTiming), "event");
-        hip_check(hipStreamCreateWithFlags(&lane2, hipStreamNonBlocking), "stream");
-        device_bytes += 2 * rbytes;
-    }
     const int L = next_lane;
-    next_lane ^= 1;
-    hipStream_t st = L ? lane2 : stream;
-    uint32_t* R0 = L ? d_R2[0] : d_R[0];
-    uint32_t* R1 = L ? d_R2[1] : d_R[1];
+    next_lane = (next_lane + 1) % lanes;
+    if (!ev_crt[L]) {
+        const size_t rbytes = (size_t)host.K * host.N * 4;
+        if (L == 0) {
+            lane_stream[0] = stream;
+            lane_R[0][0] = d_R[0];
+            lane_R[0][1] = d_R[1];
+        } else {
+            for (int i = 0; i < 2; ++i) hip_check(hipMalloc(&lane_R[L][i], rbytes), "hipMalloc residues");
+            device_bytes += 2 * rbytes;
+            hip_check(hipStreamCreateWithFlags(&lane_stream[L], hipStreamNonBlocking), "stream");
+        }
+        hip_check(hipEventCreateWithFlags(&ev_crt[L], hipEventDisableTiming), "event");
+    }
+    hipStream_t st = lane_stream[L];
+    uint32_t* R0 = lane_R[L][0];
+    uint32_t* R1 = lane_R[L][1];
     hip_check(launch_rns(dev, cfg, a, b, R0, R1, 2, st), "rns");
     hip_check(launch_col(dev, cfg, R0, R1, 2, false, st), "col");
     hip_check(launch_row(dev, cfg, R1, R0, ROW_FUSED, st), "row");
     hip_check(launch_col(dev, cfg, R1, nullptr, 1, true, st), "col inv");
-    if (crt_recorded[1 - L]) hip_check(hipStreamWaitEvent(st, ev_crt[1 - L], 0), "wait");
+    if (prev_lane >= 0 && prev_lane != L) hip_check(hipStreamWaitEvent(st, ev_crt[prev_lane], 0), "wait");
     hip_check(launch_crt(dev, cfg, R1, c, dev.G, st), "crt");
     hip_check(hipEventRecord(ev_crt[L], st), "record");
-    crt_recorded[L] = true;
+    prev_lane = L;
 }
 
 void State::sync_all() {
     hip_check(hipStreamSynchronize(stream), "synchronize");
-    if (lane2) hip_check(hipStreamSynchronize(lane2), "synchronize");
+    for (int L = 1; L < kMaxLanes; ++L)
+        if (lane_stream[L]) hip_check(hipStreamSynchronize(lane_stream[L]), "synchronize");
 }
 
 std::vector<uint32_t> State::shard_bounds(int shards) const {
@@ -236,12 +245,15 @@ void State::enqueue_sharded(const uint32_t* a, const uint32_t* b, uint32_t* c, i
 
 State::~State() {
     if (stream) (void)hipStreamSynchronize(stream);
-    if (lane2) (void)hipStreamSynchronize(lane2);
+    for (int L = 1; L < kMaxLanes; ++L)
+        if (lane_stream[L]) (void)hipStreamSynchronize(lane_stream[L]);
     for (auto& e : ev_crt)
         if (e) (void)hipEventDestroy(e);
-    for (auto& r : d_R2)
-        if (r) (void)hipFree(r);
-    if (lane2) (void)hipStreamDestroy(lane2);
+    for (int L = 1; L < kMaxLanes; ++L) {
+        for (auto& r : lane_R[L])
+            if (r) (void)hipFree(r);
+        if (lane_stream[L]) (void)hipStreamDestroy(lane_stream[L]);
+    }
     if (d_part) (void)hipFree(d_part);
     pipeline_free();
     for (int i = 0; i < 2; ++i) {

@@ -38,14 +38,17 @@ struct State {
     hipError_t enqueue_multiply_with(const uint32_t* a, const uint32_t* b, uint32_t* c, hipStream_t st,
                                      uint32_t* R0, uint32_t* R1);
 
-    // Device calls without a caller stream alternate over two lanes (the
-    // state's stream and lane2, each with its own residues), so one call's
-    // latency-bound kernels overlap the next call's. A call's CRT waits for
-    // the previous call's CRT, so results land in call order.
-    hipStream_t lane2 = nullptr;
-    uint32_t* d_R2[2] = {nullptr, nullptr};
-    hipEvent_t ev_crt[2] = {nullptr, nullptr};
-    bool crt_recorded[2] = {false, false};
+    // Device calls without a caller stream rotate over `lanes` lanes (lane 0
+    // is the state's stream; each lane has its own stream and residues), so
+    // one call's latency-bound kernels overlap the next call's. A call's CRT
+    // waits for the previous call's CRT, so results land in call order.
+    // PM_LANES (1..kMaxLanes) sets the count; lanes are created on first use.
+    static constexpr int kMaxLanes = 4;
+    int lanes = 2;
+    hipStream_t lane_stream[kMaxLanes] = {};
+    uint32_t* lane_R[kMaxLanes][2] = {};
+    hipEvent_t ev_crt[kMaxLanes] = {};
+    int prev_lane = -1;  // lane of the last call, -1 before the first
     int next_lane = 0;
     void enqueue_lanes(const uint32_t* a, const uint32_t* b, uint32_t* c);
     void sync_all();

@@ -274,7 +274,7 @@ class Multiplier:
 
     def run_device(self, da: DeviceBuffer, db: DeviceBuffer, dc: DeviceBuffer, stream=None):
         """Enqueues c = a b on device buffers and returns. Without a stream,
-        consecutive calls overlap on two internal lanes (results still land
+        consecutive calls overlap on internal lanes (results still land
         in call order); call sync() before reading dc or changing da, db."""
         _check(load().pm_run_device(self._h, da.ptr, db.ptr, dc.ptr, stream))
 
