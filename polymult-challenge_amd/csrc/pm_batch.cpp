@@ -162,7 +162,7 @@ void Runner::plan(const BatchConfig& c) {
         B.col_tc = (int)std::min<uint32_t>({(uint32_t)c.col_tc, ncols, (uint32_t)(kBatchThreads[PH_COLF] / Tc)});
         B.row_rpb = (int)std::min<uint32_t>(nrows, (uint32_t)(kBatchThreads[PH_ROW] / Tr));
         B.crt_ys = 32 * D.crt_ks + 16;
-        lds[PH_RNS] = std::max(lds[PH_RNS], (size_t)kBatchRnsTile * 4 * D.L + 32);
+        lds[PH_RNS] = std::max(lds[PH_RNS], rns_lds_bytes(kBatchRnsTile, D.L, D.rns_rt));
         lds[PH_CRT] = std::max(lds[PH_CRT], (size_t)kBatchCrtTile * B.crt_ys + (size_t)kBatchCrtTile * (D.L + 3) * 8);
         const size_t col_l = D.w1 > 4 ? ((size_t)1 << D.w1) * B.col_tc * 4 : 0;
         lds[PH_COLF] = lds[PH_COLI] = std::max(lds[PH_COLF], col_l);

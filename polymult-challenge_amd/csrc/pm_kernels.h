@@ -105,6 +105,11 @@ struct BatchTask {
 // and the column passes, 4 for the rows and the CRT).
 constexpr int kBatchThreads[PH_COUNT] = {512, 512, 256, 512, 256};
 constexpr int kBatchRnsTile = 128;
+
+// LDS of one RNS block of tn coefficients: their 4L bytes each, 32 bytes of
+// zeros read by the last k-step, and 16 bytes of per-prime constants
+// {p, pinv, bias lo, bias hi} for each of the 8 rns_rt primes of the tiles.
+inline size_t rns_lds_bytes(int tn, int L, int rns_rt) { return (size_t)tn * 4 * L + 32 + (size_t)128 * rns_rt; }
 constexpr int kBatchCrtTile = 32;
 
 // Zeroes n words (the persistent tile counters) with a kernel, so the launch

@@ -398,6 +398,9 @@ __device__ __forceinline__ long long slot_value(const v16i& acc, int g) {
 // a_n 2^32 (mod p_j), positive and below p 2^32, and one REDC gives a_n mod p_j.
 // Columns of the last k-step past 4L read the next coefficient's bytes; the
 // matching digits of D are zero.
+// |C| <= 4L 128^2 and L <= 127 (the plan's limit), so x = C0 + 256 C1 and
+// y = C2 + 256 C3 stay below 2^31 in magnitude, and the sum is
+// bias + x + 2^16 y with two 64-bit adds.
 // ---------------------------------------------------------------------------
 template <int NCT>
 __device__ __forceinline__ void rns_tile(const DevPlan& D, const u32* __restrict__ a, u32* __restrict__ R, u32 n0,
@@ -423,6 +426,17 @@ __device__ __forceinline__ void rns_tile(const DevPlan& D, const u32* __restrict
         for (int i = tid; i < words; i += nthreads) lds32[i] = (i < cnt * L ? src[i] : 0u) ^ kC;
     }
     if (tid < 8) lds32[words + tid] = 0u;  // read by the last k-step of the last column; digits there are zero
+    // Per-prime constants in LDS: the epilogue reads them without waiting on
+    // the global-memory counter, which also counts the tile's earlier stores.
+    uint4* kc = reinterpret_cast<uint4*>(lds32 + words + 8);  // 16-byte aligned: words is a multiple of 32
+    for (int j = tid; j < 8 * D.rns_rt; j += nthreads) {
+        uint4 v = make_uint4(0u, 0u, 0u, 0u);
+        if (j < K) {
+            const u64 bj = D.rns_bias[j];
+            v = make_uint4(D.p[j], D.pinv[j], (u32)bj, (u32)(bj >> 32));
+        }
+        kc[j] = v;
+    }
     __syncthreads();
 
     const int lane = tid & 63, wave = tid >> 6, nw = nthreads >> 6;
@@ -432,6 +446,11 @@ __device__ __forceinline__ void rns_tile(const DevPlan& D, const u32* __restrict
         const v4i B = {(int)q[0], (int)q[1], (int)q[2], (int)q[3]};
         return B;
     };
+    auto reduce = [&](const v16i& acc, int g, u64 bias, u32 p, u32 pinv) {
+        const int x = (int)((u32)acc[4 * g] + ((u32)acc[4 * g + 1] << 8));
+        const int y = (int)((u32)acc[4 * g + 2] + ((u32)acc[4 * g + 3] << 8));
+        return redc64(bias + (u64)(long long)x + ((u64)(long long)y << 16), p, pinv);
+    };
     for (int rt = wave; rt < D.rns_rt; rt += nw) {
         v16i acc[NCT];
 #pragma unroll
@@ -440,13 +459,17 @@ __device__ __forceinline__ void rns_tile(const DevPlan& D, const u32* __restrict
 #pragma unroll
         for (int g = 0; g < 4; ++g) {
             const int j = 8 * rt + 2 * g + half;
+            const uint4 k4 = kc[j];
             if (j >= K) continue;
-            const u32 p = D.p[j], pinv = D.pinv[j];
-            const long long bias = (long long)D.rns_bias[j];
+            const u64 bias = (u64)k4.z | ((u64)k4.w << 32);
+            u32* row = R + (size_t)j * N + n0 + col;
+            if (cnt == TN) {
 #pragma unroll
-            for (int ct = 0; ct < NCT; ++ct) {
-                const int c = 32 * ct + col;
-                if (c < cnt) R[(size_t)j * N + n0 + c] = redc64((u64)(slot_value(acc[ct], g) + bias), p, pinv);
+                for (int ct = 0; ct < NCT; ++ct) row[32 * ct] = reduce(acc[ct], g, bias, k4.x, k4.y);
+            } else {
+#pragma unroll
+                for (int ct = 0; ct < NCT; ++ct)
+                    if (32 * ct + col < cnt) row[32 * ct] = reduce(acc[ct], g, bias, k4.x, k4.y);
             }
         }
     }
@@ -824,8 +847,8 @@ LaunchCfg choose_launch(const DevPlan& D, size_t max_lds) {
     const u32 nrows = 1u << D.w1, ncols = 1u << D.w2;
     C.rns_stride = 4 * D.L;
     C.rns_tn = tile_cap("PM_RNS_TN", 128, 256);
-    while (C.rns_tn > 32 && (size_t)C.rns_tn * C.rns_stride + 32 > max_lds) C.rns_tn /= 2;
-    C.rns_lds = (size_t)C.rns_tn * C.rns_stride + 32;
+    while (C.rns_tn > 32 && rns_lds_bytes(C.rns_tn, D.L, D.rns_rt) > max_lds) C.rns_tn /= 2;
+    C.rns_lds = rns_lds_bytes(C.rns_tn, D.L, D.rns_rt);
     // 4-wave RNS blocks by default: four blocks per CU overlap their tile
     // loads better than two 8-wave blocks (PM_RNS_THREADS=512 for the latter).
     if (const char* v = std::getenv("PM_RNS_THREADS")) C.rns_threads = std::atoi(v) == 512 ? 512 : 256;

@@ -11,6 +11,9 @@ namespace pm {
 
 // Limits: N above 2^16 would need a third NTT pass.
 constexpr int kMaxLogN = 16;
+// The RNS epilogue sums its 4L-term digit products in 32 bits two at a time
+// (4L 128^2 257 < 2^31), which holds up to L = 127, i.e. W <= 4064.
+constexpr uint32_t kMaxL = 127;
 constexpr int kMaxK = 1023;
 
 struct HostPlan {

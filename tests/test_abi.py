@@ -62,6 +62,14 @@ def test_point_errors(O, pm, args, msg):
         pm.point_info(N, W, L, qv)
 
 
+def test_width_limit(pm):
+    """L above 127 is refused before any table is built (the RNS epilogue's
+    32-bit digit sums are exact up to there)."""
+    q = np.full(128, 0xFFFFFFFF, np.uint32)
+    with pytest.raises(pm.PolymultError, match="W above 4064"):
+        pm.point_info(512, 4065, 128, q)
+
+
 def test_q_limb_count_checked(O, pm):
     q = O.spec_q(64, 64)
     with pytest.raises(pm.PolymultError, match="q must have L limbs"):
