@@ -163,7 +163,7 @@ void Runner::plan(const BatchConfig& c) {
         B.row_rpb = (int)std::min<uint32_t>(nrows, (uint32_t)(kBatchThreads[PH_ROW] / Tr));
         B.crt_ys = 32 * D.crt_ks + 16;
         lds[PH_RNS] = std::max(lds[PH_RNS], rns_lds_bytes(kBatchRnsTile, D.L, D.rns_rt));
-        lds[PH_CRT] = std::max(lds[PH_CRT], (size_t)kBatchCrtTile * B.crt_ys + (size_t)kBatchCrtTile * (D.L + 3) * 8);
+        lds[PH_CRT] = std::max(lds[PH_CRT], crt_lds_bytes(kBatchCrtTile, B.crt_ys, D.L, D.crt_ks));
         const size_t col_l = D.w1 > 4 ? ((size_t)1 << D.w1) * B.col_tc * 4 : 0;
         lds[PH_COLF] = lds[PH_COLI] = std::max(lds[PH_COLF], col_l);
         const size_t row_l = D.w2 > 4 ? (size_t)B.row_rpb * (((size_t)1 << D.w2) + 16) * 4 : 0;

@@ -110,6 +110,12 @@ constexpr int kBatchRnsTile = 128;
 // zeros read by the last k-step, and 16 bytes of per-prime constants
 // {p, pinv, bias lo, bias hi} for each of the 8 rns_rt primes of the tiles.
 inline size_t rns_lds_bytes(int tn, int L, int rns_rt) { return (size_t)tn * 4 * L + 32 + (size_t)128 * rns_rt; }
+// LDS of one CRT block of tn coefficients: their y bytes (ys per coefficient),
+// the L + 3 int64 slots of each, and 16 bytes of per-prime constants
+// {G lo, G hi, p, 0} for each of the 8 crt_ks primes of the k-steps.
+inline size_t crt_lds_bytes(int tn, int ys, int L, int crt_ks) {
+    return (size_t)tn * ys + (size_t)tn * (L + 3) * 8 + (size_t)128 * crt_ks;
+}
 constexpr int kBatchCrtTile = 32;
 
 // Zeroes n words (the persistent tile counters) with a kernel, so the launch

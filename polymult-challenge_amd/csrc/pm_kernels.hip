@@ -499,12 +499,28 @@ __global__ void __launch_bounds__(512) k_rns(DevPlan D, const u32* __restrict__ 
 // k_crt_combine).
 // ---------------------------------------------------------------------------
 
+// {G lo, G hi, p, 0} of the primes j < 8 crt_ks (zeros from K on) into LDS,
+// so phase A reads them without waiting on the global-memory counter. The
+// caller syncs before phase A.
+__device__ __forceinline__ void crt_consts(const DevPlan& D, const uint2* __restrict__ G, uint4* kc, int tid,
+                                           int nthreads) {
+    for (int j = tid; j < 8 * D.crt_ks; j += nthreads) {
+        uint4 v = make_uint4(0u, 0u, 0u, 0u);
+        if (j < D.K) {
+            const uint2 g = G[j];
+            v = make_uint4(g.x, g.y, D.p[j], 0u);
+        }
+        kc[j] = v;
+    }
+}
+
 // Phase A for coefficients [n0, n0 + cnt): thread (c, r) covers primes j = r,
 // r + JS, ... of coefficient c, eight loads in flight per batch (lanes sweep
-// c, so loads coalesce).
+// c, so loads coalesce). Past cnt and from K on, the residue and the
+// constants are zero, and so is y.
 template <int TN>
-__device__ __forceinline__ void crt_y(const DevPlan& D, const u32* __restrict__ R, const uint2* __restrict__ G, u32 n0,
-                                      int cnt, int ys, unsigned char* Y, int tid, int nthreads) {
+__device__ __forceinline__ void crt_y(const DevPlan& D, const u32* __restrict__ R, const uint4* kc, u32 n0, int cnt,
+                                      int ys, unsigned char* Y, int tid, int nthreads) {
     const int K = D.K;
     const u32 N = D.N;
     const int kp = 8 * D.crt_ks;
@@ -514,20 +530,21 @@ __device__ __forceinline__ void crt_y(const DevPlan& D, const u32* __restrict__ 
     const u32* Rc = R + n0 + c;
     for (int j0 = r; j0 < kp; j0 += 8 * JS) {
         u32 v[8];
+        uint4 k[8];
 #pragma unroll
         for (int i = 0; i < 8; ++i) {
             const int j = j0 + i * JS;
             v[i] = (live && j < K) ? Rc[(size_t)j * N] : 0u;
         }
+        // all eight constant reads before the first Y write (the compiler
+        // cannot tell the two LDS regions apart and would order them)
+#pragma unroll
+        for (int i = 0; i < 8; ++i) k[i] = kc[min(j0 + i * JS, kp - 1)];
 #pragma unroll
         for (int i = 0; i < 8; ++i) {
             const int j = j0 + i * JS;
             if (j < kp) {
-                u32 y = 0;
-                if (live && j < K) {
-                    const uint2 g = G[j];
-                    y = shoup_mul(v[i], g.x, g.y, D.p[j]);
-                }
+                const u32 y = shoup_mul(v[i], k[i].x, k[i].y, k[i].z);
                 *reinterpret_cast<u32*>(Y + c * ys + 4 * j) = y ^ 0x80808080u;
             }
         }
@@ -642,7 +659,10 @@ __device__ __forceinline__ void crt_tile(const DevPlan& D, const u32* __restrict
     unsigned char* Y = lds8;  // [TN][ys]
     // slot-major, so lanes (consecutive coefficients) touch consecutive words
     long long* Pp = reinterpret_cast<long long*>(lds8 + (size_t)TN * ys);  // [nslot][TN]
-    crt_y<TN>(D, R, G, n0, cnt, ys, Y, tid, nthreads);
+    uint4* kc = reinterpret_cast<uint4*>(lds8 + (size_t)TN * ys + (size_t)(D.L + 3) * TN * 8);  // [8 crt_ks]
+    crt_consts(D, G, kc, tid, nthreads);
+    __syncthreads();
+    crt_y<TN>(D, R, kc, n0, cnt, ys, Y, tid, nthreads);
     __syncthreads();
     crt_gemm<NCT>(D, Y, ys, tid, nthreads, [&](int slot, int c, long long v) { Pp[(size_t)slot * TN + c] = v; });
     __syncthreads();
@@ -660,7 +680,10 @@ __global__ void __launch_bounds__(256) k_crt_partial(DevPlan D, const u32* __res
     const u32 n0 = blockIdx.x * (u32)TN;
     const int cnt = (int)min((u32)TN, count - n0);
     const int tid = (int)threadIdx.x;
-    crt_y<TN>(D, R, G, n0, cnt, ys, lds8, tid, (int)blockDim.x);
+    uint4* kc = reinterpret_cast<uint4*>(lds8 + (size_t)TN * ys);  // [8 crt_ks]
+    crt_consts(D, G, kc, tid, (int)blockDim.x);
+    __syncthreads();
+    crt_y<TN>(D, R, kc, n0, cnt, ys, lds8, tid, (int)blockDim.x);
     __syncthreads();
     const u32 N = D.N;
     crt_gemm<NCT>(D, lds8, ys, tid, (int)blockDim.x, [&](int slot, int c, long long v) {
@@ -857,7 +880,7 @@ LaunchCfg choose_launch(const DevPlan& D, size_t max_lds) {
     // blocks per CU hide the latency of the residue loads (measured faster at
     // N=65536).
     C.crt_tn = tile_cap("PM_CRT_TN", 32, 128);
-    auto crt_need = [&](int tn) { return (size_t)tn * C.crt_ys + (size_t)tn * (D.L + 3) * 8; };
+    auto crt_need = [&](int tn) { return crt_lds_bytes(tn, C.crt_ys, D.L, D.crt_ks); };
     while (C.crt_tn > 32 && crt_need(C.crt_tn) > max_lds) C.crt_tn /= 2;
     C.crt_lds = crt_need(C.crt_tn);
     C.col_tc = (int)std::min<u32>(ncols, 32);
@@ -930,7 +953,7 @@ hipError_t launch_crt(const DevPlan& D, const LaunchCfg& C, const u32* R, u32* o
 
 hipError_t launch_crt_partial(const DevPlan& D, const LaunchCfg& C, const u32* R, long long* P, hipStream_t s) {
     const dim3 grid((D.N + C.crt_tn - 1) / C.crt_tn), block(256);
-    const size_t lds = (size_t)C.crt_tn * C.crt_ys;
+    const size_t lds = (size_t)C.crt_tn * C.crt_ys + (size_t)128 * D.crt_ks;
     switch (C.crt_tn) {
         case 128: hipLaunchKernelGGL((k_crt_partial<4>), grid, block, lds, s, D, R, D.G, P, D.N, C.crt_ys); break;
         case 64: hipLaunchKernelGGL((k_crt_partial<2>), grid, block, lds, s, D, R, D.G, P, D.N, C.crt_ys); break;
