@@ -164,7 +164,7 @@ void Runner::plan(const BatchConfig& c) {
         B.crt_ys = 32 * D.crt_ks + 16;
         lds[PH_RNS] = std::max(lds[PH_RNS], rns_lds_bytes(kBatchRnsTile, D.L, D.rns_rt));
         lds[PH_CRT] = std::max(lds[PH_CRT], crt_lds_bytes(kBatchCrtTile, B.crt_ys, D.L, D.crt_ks));
-        const size_t col_l = D.w1 > 4 ? ((size_t)1 << D.w1) * B.col_tc * 4 : 0;
+        const size_t col_l = col_lds_bytes(D.w1, B.col_tc);
         lds[PH_COLF] = lds[PH_COLI] = std::max(lds[PH_COLF], col_l);
         const size_t row_l = D.w2 > 4 ? (size_t)B.row_rpb * (((size_t)1 << D.w2) + 16) * 4 : 0;
         lds[PH_ROW] = std::max(lds[PH_ROW], row_l);

@@ -109,6 +109,9 @@ constexpr int kBatchRnsTile = 128;
 // LDS of one RNS block of tn coefficients: their 4L bytes each, 32 bytes of
 // zeros read by the last k-step, and 16 bytes of per-prime constants
 // {p, pinv, bias lo, bias hi} for each of the 8 rns_rt primes of the tiles.
+// LDS of one column block (w1 > 4 only): the [2^w1][tc] transpose tile and
+// the first 2^w1 twiddle pairs of the prime.
+inline size_t col_lds_bytes(int w1, int tc) { return w1 > 4 ? ((size_t)1 << w1) * ((size_t)tc * 4 + 8) : 0; }
 inline size_t rns_lds_bytes(int tn, int L, int rns_rt) { return (size_t)tn * 4 * L + 32 + (size_t)128 * rns_rt; }
 // LDS of one CRT block of tn coefficients: their y bytes (ys per coefficient),
 // the L + 3 int64 slots of each, and 16 bytes of per-prime constants

@@ -142,8 +142,11 @@ __device__ __forceinline__ void inv_pass1(u32 (&x)[Line<NB>::E], const uint2* __
 // ---------------------------------------------------------------------------
 // Column tile: columns [col0, col0 + tc) of prime j, tc T threads (threads
 // from tc T on only take part in the barriers). LDS tile [2^NB][tc] for the
-// transpose between the two register passes. Threads sweep columns fastest,
-// so every global access of a wave reads or writes whole row segments.
+// transpose between the two register passes, then the table's first 2^NB
+// twiddle pairs: pass 2 reads its per-thread twiddles there (staged before
+// pass 1, behind the transpose barrier), not from global memory stage by
+// stage. Threads sweep columns fastest, so every global access of a wave
+// reads or writes whole row segments.
 // ---------------------------------------------------------------------------
 template <int NB, bool INV>
 __device__ __forceinline__ void col_tile(const DevPlan& D, u32* __restrict__ R, int j, int col0, int tc, u32* lds,
@@ -157,12 +160,18 @@ __device__ __forceinline__ void col_tile(const DevPlan& D, u32* __restrict__ R, 
     const size_t ncols = (size_t)1 << D.w2;
     u32* col = R + (size_t)j * D.N + col0 + c;
     u32 x[1][E];
+    uint2* ltw = reinterpret_cast<uint2*>(lds + (1 << NB) * tc);
+    auto stage_twiddles = [&] {
+        if (on)
+            for (int i = tid; i < (1 << NB); i += tc * T) ltw[i] = tab[i];
+    };
     if (!INV) {
         if (on) {
 #pragma unroll
             for (int k = 0; k < E; ++k) x[0][k] = col[(size_t)(t + k * T) * ncols];
-            fwd_pass1<NB, 1>(x, tab, 0, 0, p);
         }
+        if constexpr (R2 > 0) stage_twiddles();  // read after the transpose barrier
+        if (on) fwd_pass1<NB, 1>(x, tab, 0, 0, p);
         if constexpr (R2 > 0) {
             if (on) {
 #pragma unroll
@@ -172,7 +181,7 @@ __device__ __forceinline__ void col_tile(const DevPlan& D, u32* __restrict__ R, 
             if (on) {
 #pragma unroll
                 for (int k = 0; k < E; ++k) x[0][k] = lds[(t * E + k) * tc + c];
-                fwd_pass2<NB, 1>(x, tab, 0, 0, t, p);
+                fwd_pass2<NB, 1>(x, ltw, 0, 0, t, p);
 #pragma unroll
                 for (int k = 0; k < E; ++k) col[(size_t)(t * E + k) * ncols] = x[0][k];
             }
@@ -185,7 +194,11 @@ __device__ __forceinline__ void col_tile(const DevPlan& D, u32* __restrict__ R, 
             if (on) {
 #pragma unroll
                 for (int k = 0; k < E; ++k) x[0][k] = col[(size_t)(t * E + k) * ncols];
-                inv_pass2<NB>(x[0], tab, 0, 0, t, p);
+            }
+            stage_twiddles();
+            __syncthreads();
+            if (on) {
+                inv_pass2<NB>(x[0], ltw, 0, 0, t, p);
 #pragma unroll
                 for (int k = 0; k < E; ++k) lds[(t * E + k) * tc + c] = x[0][k];
             }
@@ -916,7 +929,7 @@ hipError_t launch_col(const DevPlan& D, const LaunchCfg& C, u32* R0, u32* R1, in
     const u32 ncols = 1u << D.w2;
     const int T = line_threads(D.w1);
     const dim3 grid(ncols / C.col_tc, D.K, nops), block(C.col_tc * T);
-    const size_t lds = D.w1 > 4 ? ((size_t)1 << D.w1) * C.col_tc * 4 : 0;
+    const size_t lds = col_lds_bytes(D.w1, C.col_tc);
     return inverse ? col_dispatch<true>(D.w1, grid, block, lds, s, D, R0, R1, C.col_tc)
                    : col_dispatch<false>(D.w1, grid, block, lds, s, D, R0, R1, C.col_tc);
 }
