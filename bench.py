@@ -109,6 +109,7 @@ def batched(pmlib, O, count: int, points, seed: int, check: int, iters: int, tun
     for i in range(min(check, count)):
         n, w, q = prods[i]
         exact &= bool(np.array_equal(dc[i].download((n, O.limbs_of(w))), O.OraclePlan(n, w, q).multiply(*ops[i])))
+    B.time(da, db, dc, 10)  # warm-up: the clocks settle after the oracle check
     ms = B.time(da, db, dc, iters)
     B.close()
     # the plain per-call path on the same products: one device-resident
@@ -117,14 +118,14 @@ def batched(pmlib, O, count: int, points, seed: int, check: int, iters: int, tun
     for (n, w, q), a_, b_, c_ in zip(prods, da, db, dc):
         if (n, w) not in plain_ms:
             with pmlib.Multiplier(n, w, O.q_limbs(q, O.limbs_of(w))) as m:
-                for _ in range(3):
+                for _ in range(50):
                     m.run_device(a_, b_, c_)
                 m.sync()
                 t = time.perf_counter()
-                for _ in range(10):
+                for _ in range(100):
                     m.run_device(a_, b_, c_)
                 m.sync()
-                plain_ms[(n, w)] = 1e3 * (time.perf_counter() - t) / 10
+                plain_ms[(n, w)] = 1e3 * (time.perf_counter() - t) / 100
     plain_total = sum(plain_ms[(n, w)] for n, w, _ in prods)
     io = sum(12 * n * O.limbs_of(w) for n, w in points)
     return {"products": count, "ms_per_launch": round(ms, 4), "mults_per_s": round(count / (ms * 1e-3), 1),
@@ -303,10 +304,10 @@ def main():
     batch_same = batch_mix = None
     if "batch" in legs:
         tune = not args.no_autotune
-        batch_same = batched(pmlib, O, 16, [(N, W)] * 16, SEED + 100, 2, 20, tune)
+        batch_same = batched(pmlib, O, 16, [(N, W)] * 16, SEED + 100, 2, 50, tune)
         rng = np.random.default_rng(SEED)
         mix = [(int(2 ** rng.integers(11, 17)), int(rng.integers(128, 2049))) for _ in range(64)]
-        batch_mix = batched(pmlib, O, 64, mix, SEED + 200, 4, 20, tune)
+        batch_mix = batched(pmlib, O, 64, mix, SEED + 200, 4, 50, tune)
         batch_mix["workload"] = "64 products, N log-uniform in 2^11..2^16, W uniform in [128, 2048], seed %d" % SEED
 
     serve = serving(pmlib, O, (200, 1000, 5000), args.serve_seconds) if "serve" in legs else None
