@@ -109,6 +109,12 @@ void State::init(uint32_t N, uint32_t W, uint32_t L, const uint32_t* q, size_t q
     hip_check(hipDeviceGetAttribute(&max_lds, hipDeviceAttributeMaxSharedMemoryPerBlock, device), "device attribute");
     hip_check(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking), "stream");
     if (const char* e = std::getenv("PM_LANES")) lanes = std::min(kMaxLanes, std::max(1, std::atoi(e)));
+    // The lane streams are created right after the state's stream (their
+    // residues only on first use): streams share the process's few hardware
+    // queues, and streams created back to back land on different ones, so
+    // the lanes really run side by side.
+    for (int L = 1; L < lanes; ++L)
+        hip_check(hipStreamCreateWithFlags(&lane_stream[L], hipStreamNonBlocking), "stream");
     upload_plan();
     cfg = choose_launch(dev, std::min<size_t>((size_t)max_lds, 64 * 1024));
     if (cfg.crt_lds > (size_t)max_lds || cfg.rns_lds > (size_t)max_lds)
@@ -155,7 +161,6 @@ void State::enqueue_lanes(const uint32_t* a, const uint32_t* b, uint32_t* c) {
         } else {
             for (int i = 0; i < 2; ++i) hip_check(hipMalloc(&lane_R[L][i], rbytes), "hipMalloc residues");
             device_bytes += 2 * rbytes;
-            hip_check(hipStreamCreateWithFlags(&lane_stream[L], hipStreamNonBlocking), "stream");
         }
         hip_check(hipEventCreateWithFlags(&ev_crt[L], hipEventDisableTiming), "event");
     }

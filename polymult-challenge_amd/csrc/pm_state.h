@@ -42,7 +42,8 @@ struct State {
     // is the state's stream; each lane has its own stream and residues), so
     // one call's latency-bound kernels overlap the next call's. A call's CRT
     // waits for the previous call's CRT, so results land in call order.
-    // PM_LANES (1..kMaxLanes) sets the count; lanes are created on first use.
+    // PM_LANES (1..kMaxLanes) sets the count. The lane streams are created
+    // with the state's stream, their residues on first use.
     static constexpr int kMaxLanes = 4;
     int lanes = 2;
     hipStream_t lane_stream[kMaxLanes] = {};
