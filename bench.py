@@ -118,14 +118,20 @@ def batched(pmlib, O, count: int, points, seed: int, check: int, iters: int, tun
     for (n, w, q), a_, b_, c_ in zip(prods, da, db, dc):
         if (n, w) not in plain_ms:
             with pmlib.Multiplier(n, w, O.q_limbs(q, O.limbs_of(w))) as m:
-                for _ in range(50):
-                    m.run_device(a_, b_, c_)
-                m.sync()
+                # by time, whatever the size: 30 ms of warm-up (clocks), then
+                # at least 100 ms and 20 calls timed
+                def calls(k):
+                    for _ in range(k):
+                        m.run_device(a_, b_, c_)
+                    m.sync()
                 t = time.perf_counter()
-                for _ in range(100):
-                    m.run_device(a_, b_, c_)
-                m.sync()
-                plain_ms[(n, w)] = 1e3 * (time.perf_counter() - t) / 100
+                while time.perf_counter() - t < 0.03:
+                    calls(10)
+                k, t = 0, time.perf_counter()
+                while k < 20 or time.perf_counter() - t < 0.1:
+                    calls(10)
+                    k += 10
+                plain_ms[(n, w)] = 1e3 * (time.perf_counter() - t) / k
     plain_total = sum(plain_ms[(n, w)] for n, w, _ in prods)
     io = sum(12 * n * O.limbs_of(w) for n, w in points)
     return {"products": count, "ms_per_launch": round(ms, 4), "mults_per_s": round(count / (ms * 1e-3), 1),

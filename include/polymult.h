@@ -77,10 +77,11 @@ int pm_copy_to_host(void* dst, const void* src, size_t bytes);
 /* Waits for every device call of the state (every lane). */
 int pm_sync(pm_state* s);
 
-/* Runs `iters` device-resident multiplies back to back on the state's stream
- * and reports, from HIP events on that stream, the average milliseconds of
- * the whole multiply and of each kernel: ms[0] whole, ms[1] RNS, ms[2]
- * column forward, ms[3] fused rows, ms[4] column inverse, ms[5] CRT. */
+/* Average milliseconds, from HIP events on the state's stream, of `iters`
+ * launches enqueued back to back (after one warm launch): ms[1] RNS, ms[2]
+ * column forward, ms[3] fused rows, ms[4] column inverse, ms[5] CRT (each
+ * kernel repeated alone, on the residues the chain leaves), then ms[0] the
+ * whole multiply. d_c holds a * b afterwards. */
 int pm_time_device(pm_state* s, const uint32_t* d_a, const uint32_t* d_b, uint32_t* d_c, int iters, float* ms);
 
 /* PCIe and host-copy rates (GB/s, best of `reps`) for transfers of `bytes`:
