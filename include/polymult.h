@@ -74,6 +74,11 @@ int pm_dev_alloc(size_t bytes, void** out);
 int pm_dev_free(void* p);
 int pm_copy_to_device(void* dst, const void* src, size_t bytes);
 int pm_copy_to_host(void* dst, const void* src, size_t bytes);
+/* A hipStream_t (non-blocking, default priority) to pass as `stream` to the
+ * device entry points, its host-side wait, and its release. */
+int pm_stream_create(void** out);
+int pm_stream_sync(void* stream);
+int pm_stream_destroy(void* stream);
 /* Waits for every device call of the state (every lane). */
 int pm_sync(pm_state* s);
 

@@ -351,6 +351,22 @@ int pm_dev_free(void* p) {
 int pm_copy_to_device(void* dst, const void* src, size_t bytes) {
     return guarded([&] { hip_check(hipMemcpy(dst, src, bytes, hipMemcpyHostToDevice), "copy to device"); });
 }
+int pm_stream_create(void** out) {
+    return guarded([&] {
+        if (!out) throw std::runtime_error("null argument");
+        hipStream_t s = nullptr;
+        hip_check(hipStreamCreateWithFlags(&s, hipStreamNonBlocking), "stream");
+        *out = s;
+    });
+}
+int pm_stream_sync(void* stream) {
+    return guarded([&] { hip_check(hipStreamSynchronize(static_cast<hipStream_t>(stream)), "synchronize"); });
+}
+int pm_stream_destroy(void* stream) {
+    return guarded([&] {
+        if (stream) hip_check(hipStreamDestroy(static_cast<hipStream_t>(stream)), "stream");
+    });
+}
 int pm_copy_to_host(void* dst, const void* src, size_t bytes) {
     return guarded([&] { hip_check(hipMemcpy(dst, src, bytes, hipMemcpyDeviceToHost), "copy to host"); });
 }

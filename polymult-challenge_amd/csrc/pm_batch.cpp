@@ -312,6 +312,8 @@ struct Batch {
     static constexpr int kMaxBranches = 3;
     hipStream_t side[kMaxBranches - 1] = {};  // further branches, used only inside the capture
     hipEvent_t fork = nullptr, join[kMaxBranches - 1] = {};
+    hipEvent_t launched = nullptr;  // recorded after each graph launch, on its stream
+    bool any_launch = false;
     hipGraph_t graph = nullptr;
     hipGraphExec_t exec = nullptr;
 
@@ -319,7 +321,9 @@ struct Batch {
         if (stream) (void)hipStreamSynchronize(stream);
         drop_graph();
         if (d_R) (void)hipFree(d_R);
+        if (launched) (void)hipEventSynchronize(launched);
         if (fork) (void)hipEventDestroy(fork);
+        if (launched) (void)hipEventDestroy(launched);
         for (auto e : join)
             if (e) (void)hipEventDestroy(e);
         for (auto t : side)
@@ -356,13 +360,18 @@ struct Batch {
             }
         }
         if (!changed) return;
+        // The last launch, on whatever stream it went, may still read d_prods
+        // and run its graph: the upload waits for it on the device.
+        if (any_launch) hip_check(hipStreamWaitEvent(stream, launched, 0), "wait");
         hip_check(hipMemcpyAsync(run.d_prods, run.prods.data(), run.prods.size() * sizeof(BatchProduct),
                                  hipMemcpyHostToDevice, stream),
                   "upload products");
         // The phase kernels read the pointers from d_prods, but a group of one
-        // product was recorded with them as kernel arguments: capture again.
+        // product was recorded with them as kernel arguments: capture again,
+        // once the graph in flight has finished.
         for (int g : run.single)
             if (g >= 0) {
+                if (any_launch) hip_check(hipEventSynchronize(launched), "wait");
                 drop_graph();
                 break;
             }
@@ -393,6 +402,8 @@ struct Batch {
             hip_check(hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0), "graph instantiate");
         }
         hip_check(hipGraphLaunch(exec, st), "graph launch");
+        hip_check(hipEventRecord(launched, st), "record");
+        any_launch = true;
     }
     float time_ms(int iters) {
         hipEvent_t e0, e1;
@@ -418,6 +429,7 @@ void batch_build(Batch& B, uint32_t count, const uint32_t* N, const uint32_t* W,
     hip_check(hipStreamCreateWithFlags(&B.stream, hipStreamNonBlocking), "stream");
     for (auto& t : B.side) hip_check(hipStreamCreateWithFlags(&t, hipStreamNonBlocking), "stream");
     hip_check(hipEventCreateWithFlags(&B.fork, hipEventDisableTiming), "event");
+    hip_check(hipEventCreateWithFlags(&B.launched, hipEventDisableTiming), "event");
     for (auto& e : B.join) hip_check(hipEventCreateWithFlags(&e, hipEventDisableTiming), "event");
     size_t rwords = 0;
     for (size_t i = 0; i < count; ++i) rwords += B.cat.residue_words(i);

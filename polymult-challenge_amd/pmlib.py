@@ -21,7 +21,8 @@ LIB_PATH = HERE / "libpolymult.so"
 # Every C symbol include/polymult.h declares.
 C_SYMBOLS = (
     "pm_init", "pm_run", "pm_run_device", "pm_free", "pm_last_error", "pm_query", "pm_point_info",
-    "pm_dev_alloc", "pm_dev_free", "pm_copy_to_device", "pm_copy_to_host", "pm_sync", "pm_time_device",
+    "pm_dev_alloc", "pm_dev_free", "pm_copy_to_device", "pm_copy_to_host", "pm_stream_create", "pm_stream_sync",
+    "pm_stream_destroy", "pm_sync", "pm_time_device",
     "pm_shard_bounds", "pm_run_device_sharded", "pm_time_shard",
     "pm_stage_rns", "pm_stage_ntt_fwd", "pm_stage_ntt_inv", "pm_stage_crt", "pm_probe_transfers", "pm_probe_copy",
     "pm_batch_create", "pm_batch_free", "pm_batch_last_error", "pm_batch_product_info", "pm_batch_run",
@@ -110,6 +111,9 @@ def load() -> ctypes.CDLL:
         "pm_dev_free": ([_vp], ctypes.c_int),
         "pm_copy_to_device": ([_vp, _vp, ctypes.c_size_t], ctypes.c_int),
         "pm_copy_to_host": ([_vp, _vp, ctypes.c_size_t], ctypes.c_int),
+        "pm_stream_create": ([ctypes.POINTER(_vp)], ctypes.c_int),
+        "pm_stream_sync": ([_vp], ctypes.c_int),
+        "pm_stream_destroy": ([_vp], ctypes.c_int),
         "pm_sync": ([_vp], ctypes.c_int),
         "pm_time_device": ([_vp, _vp, _vp, _vp, ctypes.c_int, ctypes.POINTER(ctypes.c_float)], ctypes.c_int),
         "pm_stage_rns": ([_vp, _vp, _vp], ctypes.c_int),
@@ -229,6 +233,29 @@ class DeviceBuffer:
     def __del__(self):
         try:
             self.free()
+        except Exception:
+            pass
+
+
+class Stream:
+    """A hipStream_t from libpolymult's own HIP runtime, to pass as `stream`."""
+
+    def __init__(self):
+        p = _vp()
+        _check(load().pm_stream_create(ctypes.byref(p)))
+        self.handle = p
+
+    def sync(self):
+        _check(load().pm_stream_sync(self.handle))
+
+    def close(self):
+        if self.handle:
+            _check(load().pm_stream_destroy(self.handle))
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
         except Exception:
             pass
 

@@ -129,3 +129,28 @@ def test_batch_new_buffers_each_launch(O, pm):
         assert np.array_equal(dc2[0].download((N, O.limbs_of(W))), m.run(*ops2[0]))
     # and the first buffers were left alone by the second launch
     _check(O, prods, ops, dc, idx=[1, 2])
+
+
+@pytest.mark.parametrize("group_mb", [96, 0])
+def test_batch_back_to_back_on_caller_stream(O, pm, group_mb):
+    """Launches on a caller's stream, alternating two buffer sets with no host
+    wait between them: each launch computes on its own buffers (the product
+    table is rewritten only after the launch before has finished with it).
+    group_mb 96: single-product groups (recaptured); 0: one group whose phase
+    kernels read the table."""
+    points = [(4096, 256), (2048, 868), (8192, 438), (512, 1747)]
+    print("back-to-back launches, group_mb %d" % group_mb, flush=True)
+    B, prods, ops, da, db, dc = _batch(O, pm, points, 800, "uniform")
+    B.configure(col_tile=32, ceil_split=0, blocks_per_cu=0, group_mb=group_mb, branches=2)
+    ops2 = [O.operands("mixed", N, W, q, 900 + i) for i, (N, W, q) in enumerate(prods)]
+    da2 = [pm.DeviceBuffer.from_array(a) for a, _ in ops2]
+    db2 = [pm.DeviceBuffer.from_array(b) for _, b in ops2]
+    dc2 = [pm.DeviceBuffer(N * O.limbs_of(W) * 4) for N, W in points]
+    s = pm.Stream()
+    for _ in range(3):
+        B.run(da, db, dc, stream=s.handle)
+        B.run(da2, db2, dc2, stream=s.handle)
+    s.sync()
+    _check(O, prods, ops, dc)
+    _check(O, prods, ops2, dc2)
+    s.close()

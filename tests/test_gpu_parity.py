@@ -183,3 +183,18 @@ def test_device_calls_in_flight(O, pm):
         assert np.array_equal(shared.download((N, L)), dc[-1].download((N, L)))
         # the host-buffer path on the same state afterwards
         assert np.array_equal(m.run(*ops[1]), dc[1].download((N, L)))
+
+
+def test_time_device_leaves_product(O, pm):
+    """pm_time_device repeats each kernel on the chain's residues, then runs
+    whole multiplies: its times are positive and d_c ends holding a * b."""
+    N, W = 4096, 256
+    q, L, m, P = _setup(O, pm, N, W)
+    a, b = O.operands("mixed", N, W, q, 77)
+    with m:
+        da, db = pm.DeviceBuffer.from_array(a), pm.DeviceBuffer.from_array(b)
+        dc = pm.DeviceBuffer(N * L * 4)
+        ms = m.time_device(da, db, dc, 5)
+        assert len(ms) == 6 and all(t > 0 for t in ms)
+        assert ms[0] >= max(ms[1:])
+        assert np.array_equal(dc.download((N, L)), P.multiply(a, b))
