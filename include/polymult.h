@@ -53,7 +53,9 @@ int pm_run(pm_state* s, const uint32_t* a, const uint32_t* b, uint32_t* c);
  * (two by default, PM_LANES=1..4; each with its own residue buffers), so one
  * call's kernels overlap the next call's; results still land in call order
  * (a call's final kernel waits for the previous call's), and pm_sync waits
- * for every lane. Inputs
+ * for every lane. Calls on different streams (and pm_run, the sharded call)
+ * share the state's default residue buffers: each waits on the device for
+ * the one before, so they stay correct but do not overlap. Inputs
  * must not change before pm_sync. No counterpart in the reference: this is
  * the device-resident form of the same call. */
 int pm_run_device(pm_state* s, const uint32_t* d_a, const uint32_t* d_b, uint32_t* d_c, void* stream);

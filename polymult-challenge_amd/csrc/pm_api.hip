@@ -109,6 +109,7 @@ void State::init(uint32_t N, uint32_t W, uint32_t L, const uint32_t* q, size_t q
     int max_lds = 0;
     hip_check(hipDeviceGetAttribute(&max_lds, hipDeviceAttributeMaxSharedMemoryPerBlock, device), "device attribute");
     hip_check(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking), "stream");
+    hip_check(hipEventCreateWithFlags(&ev_R, hipEventDisableTiming), "event");
     if (const char* e = std::getenv("PM_LANES")) lanes = std::min(kMaxLanes, std::max(1, std::atoi(e)));
     // The lane streams are created right after the state's stream (their
     // residues only on first use): streams share the process's few hardware
@@ -134,6 +135,15 @@ void State::init(uint32_t N, uint32_t W, uint32_t L, const uint32_t* q, size_t q
     hip_check(hipMemsetAsync(d_in[1], 0, coeff_words * 4, stream), "memset");
     hip_check(enqueue_multiply(d_in[0], d_in[1], d_out, stream), "warm-up multiply");
     hip_check(hipStreamSynchronize(stream), "warm-up");
+}
+
+void State::acquire_R(hipStream_t st) {
+    if (R_used) hip_check(hipStreamWaitEvent(st, ev_R, 0), "wait");
+}
+
+void State::release_R(hipStream_t st) {
+    hip_check(hipEventRecord(ev_R, st), "record");
+    R_used = true;
 }
 
 hipError_t State::enqueue_multiply(const uint32_t* a, const uint32_t* b, uint32_t* c, hipStream_t st) {
@@ -168,6 +178,7 @@ void State::enqueue_lanes(const uint32_t* a, const uint32_t* b, uint32_t* c) {
     hipStream_t st = lane_stream[L];
     uint32_t* R0 = lane_R[L][0];
     uint32_t* R1 = lane_R[L][1];
+    if (L == 0) acquire_R(st);  // lane 0 works in d_R
     hip_check(launch_rns(dev, cfg, a, b, R0, R1, 2, st), "rns");
     hip_check(launch_col(dev, cfg, R0, R1, 2, false, st), "col");
     hip_check(launch_row(dev, cfg, R1, R0, ROW_FUSED, st), "row");
@@ -175,6 +186,7 @@ void State::enqueue_lanes(const uint32_t* a, const uint32_t* b, uint32_t* c) {
     if (prev_lane >= 0 && prev_lane != L) hip_check(hipStreamWaitEvent(st, ev_crt[prev_lane], 0), "wait");
     hip_check(launch_crt(dev, cfg, R1, c, dev.G, st), "crt");
     hip_check(hipEventRecord(ev_crt[L], st), "record");
+    if (L == 0) release_R(st);
     prev_lane = L;
 }
 
@@ -231,7 +243,10 @@ void State::enqueue_sharded(const uint32_t* a, const uint32_t* b, uint32_t* c, i
     const std::vector<uint32_t> bd = shard_bounds(shards);
     const size_t stride = (size_t)(dev.L + 3) * dev.N;  // int64 slots per shard
     const size_t need = (size_t)shards * stride * 8;
+    acquire_R(st);
     if (need > part_bytes) {
+        // the partials are reallocated: every earlier use has to be done
+        if (R_used) hip_check(hipEventSynchronize(ev_R), "sync");
         hip_check(hipStreamSynchronize(st), "sync");
         if (d_part) hip_check(hipFree(d_part), "hipFree");
         d_part = nullptr;
@@ -247,9 +262,11 @@ void State::enqueue_sharded(const uint32_t* a, const uint32_t* b, uint32_t* c, i
         hip_check(enqueue_shard(V, C, bd[g], a, b, d_part + (size_t)g * stride, st), "shard");
     }
     hip_check(launch_crt_combine(dev, cfg, d_part, shards, stride, c, st), "combine");
+    release_R(st);
 }
 
 State::~State() {
+    if (R_used) (void)hipEventSynchronize(ev_R);
     if (stream) (void)hipStreamSynchronize(stream);
     for (int L = 1; L < kMaxLanes; ++L)
         if (lane_stream[L]) (void)hipStreamSynchronize(lane_stream[L]);
@@ -268,6 +285,7 @@ State::~State() {
     }
     if (d_out) (void)hipFree(d_out);
     if (d_tables) (void)hipFree(d_tables);
+    if (ev_R) (void)hipEventDestroy(ev_R);
     if (stream) (void)hipStreamDestroy(stream);
 }
 
@@ -331,9 +349,12 @@ int pm_run(pm_state* s, const uint32_t* a, const uint32_t* b, uint32_t* c) {
 int pm_run_device(pm_state* s, const uint32_t* d_a, const uint32_t* d_b, uint32_t* d_c, void* stream) {
     return guarded([&] {
         if (!s || !d_a || !d_b || !d_c) throw std::runtime_error("null argument");
-        if (stream)
-            hip_check(s->s->enqueue_multiply(d_a, d_b, d_c, static_cast<hipStream_t>(stream)), "multiply");
-        else
+        if (stream) {
+            hipStream_t st = static_cast<hipStream_t>(stream);
+            s->s->acquire_R(st);
+            hip_check(s->s->enqueue_multiply(d_a, d_b, d_c, st), "multiply");
+            s->s->release_R(st);
+        } else
             s->s->enqueue_lanes(d_a, d_b, d_c);
     });
 }
@@ -381,6 +402,7 @@ int pm_time_device(pm_state* h, const uint32_t* d_a, const uint32_t* d_b, uint32
         hipEvent_t e0, e1;
         hip_check(hipEventCreate(&e0), "event");
         hip_check(hipEventCreate(&e1), "event");
+        S.acquire_R(st);
         // Average of n launches enqueued back to back between two events: an
         // event between every pair of kernels would add its own gap to each.
         auto time_runs = [&](const std::function<void()>& launch) {
@@ -402,6 +424,7 @@ int pm_time_device(pm_state* h, const uint32_t* d_a, const uint32_t* d_b, uint32
         ms[5] = time_runs([&] { hip_check(pm::launch_crt(D, C, S.d_R[1], d_c, D.G, st), "crt"); });
         // Whole multiplies last, so d_c ends holding a * b.
         ms[0] = time_runs([&] { hip_check(S.enqueue_multiply(d_a, d_b, d_c, st), "multiply"); });
+        S.release_R(st);
         (void)hipEventDestroy(e0);
         (void)hipEventDestroy(e1);
     });
@@ -428,7 +451,7 @@ int pm_time_shard(pm_state* h, const uint32_t* d_a, const uint32_t* d_b, uint32_
     return guarded([&] {
         pm::State& S = *h->s;
         hipStream_t st = S.stream;
-        S.enqueue_sharded(d_a, d_b, d_c, shards, st);  // sizes the partials, warms up
+        S.enqueue_sharded(d_a, d_b, d_c, shards, st);  // sizes the partials, warms up; orders after earlier uses
         hip_check(hipStreamSynchronize(st), "sync");
         const std::vector<uint32_t> bd = S.shard_bounds(shards);
         int max_lds = 0;
@@ -458,6 +481,7 @@ int pm_time_shard(pm_state* h, const uint32_t* d_a, const uint32_t* d_b, uint32_
             hip_check(pm::launch_crt_partial(V, C, R1, S.d_part, st), "crt partial");
             hip_check(hipEventRecord(ev[6], st), "record");
             hip_check(pm::launch_crt_combine(S.dev, S.cfg, S.d_part, shards, stride, d_c, st), "combine");
+            S.release_R(st);
             hip_check(hipEventRecord(ev[7], st), "record");
             hip_check(hipEventSynchronize(ev[7]), "sync");
             for (int k = 1; k < NE; ++k) {

@@ -269,6 +269,7 @@ void State::run_host(const uint32_t* a, const uint32_t* b, uint32_t* c) {
     };
 
     // ---- input: stage, H2D and RNS per chunk ----
+    acquire_R(stream);  // d_R: after any caller-stream or sharded call still in flight
     const int nin = (int)P.cin.size();
     if (P.profile) hip_check(hipEventRecord(P.tl[0], P.s_in), "record");
     for (int op = 0; op < 2; ++op) {
@@ -312,6 +313,7 @@ void State::run_host(const uint32_t* a, const uint32_t* b, uint32_t* c) {
         hip_check(launch_crt(dev, cfg, d_R[1], d_out, dev.G, stream, first_n, host.N - first_n), "crt");
         hip_check(hipEventRecord(P.ev_crt[1], stream), "record");
     }
+    release_R(stream);
     for (int k = 0; k < nout; ++k) {
         const auto ch = P.cout[k];
         const size_t off = (size_t)ch.first * L, words = (size_t)ch.count * L;

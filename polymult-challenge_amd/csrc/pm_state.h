@@ -30,6 +30,14 @@ struct State {
     Pipeline* pipe = nullptr;
     long long* d_part = nullptr;  // prime-shard CRT partials, grown on demand
     size_t part_bytes = 0;
+    // d_R and d_part serve the caller-stream calls, lane 0, the host path, the
+    // sharded path and the timing helpers, on whatever stream each runs. Every
+    // use waits on the device for the one before (acquire_R) and marks its
+    // own end (release_R).
+    hipEvent_t ev_R = nullptr;
+    bool R_used = false;
+    void acquire_R(hipStream_t st);
+    void release_R(hipStream_t st);
 
     void init(uint32_t N, uint32_t W, uint32_t L, const uint32_t* q, size_t q_limbs);
     void upload_plan();

@@ -198,3 +198,36 @@ def test_time_device_leaves_product(O, pm):
         assert len(ms) == 6 and all(t > 0 for t in ms)
         assert ms[0] >= max(ms[1:])
         assert np.array_equal(dc.download((N, L)), P.multiply(a, b))
+
+
+def test_device_calls_across_streams(O, pm):
+    """One state driven from two caller streams, the default stream (lanes),
+    the sharded path and the host path, interleaved with no host wait: every
+    result is exact (the state's shared residues are ordered across streams)."""
+    N, W = 8192, 438
+    q = O.spec_q(N, W)
+    L = O.limbs_of(W)
+    ops = [O.operands(k, N, W, q, 1200 + i) for i, k in enumerate(["uniform", "mixed", "max", "small16", "sparse"])]
+    want = [O.OraclePlan(N, W, q).multiply(a, b) for a, b in ops]
+    s1, s2 = pm.Stream(), pm.Stream()
+    with pm.Multiplier(N, W, O.q_limbs(q, L)) as m:
+        da = [pm.DeviceBuffer.from_array(a) for a, _ in ops]
+        db = [pm.DeviceBuffer.from_array(b) for _, b in ops]
+        dc = [pm.DeviceBuffer(N * L * 4) for _ in range(8)]
+        m.run_device(da[0], db[0], dc[0], s1.handle)
+        m.run_device(da[1], db[1], dc[1], s2.handle)
+        m.run_device(da[2], db[2], dc[2])
+        m.run_device(da[3], db[3], dc[3], s1.handle)
+        m.run_device_sharded(da[4], db[4], dc[4], 2, s2.handle)
+        c_host = m.run(*ops[1])
+        m.run_device(da[2], db[2], dc[5], s2.handle)
+        m.run_device(da[0], db[0], dc[6])
+        m.run_device(da[3], db[3], dc[7], s1.handle)
+        s1.sync()
+        s2.sync()
+        m.sync()
+        assert np.array_equal(c_host, want[1])
+        for k, i in enumerate([0, 1, 2, 3, 4, 2, 0, 3]):
+            assert np.array_equal(dc[k].download((N, L)), want[i]), k
+    s1.close()
+    s2.close()
