@@ -145,7 +145,30 @@ SERVE_POINTS = [(8192, 438), (32768, 868), (65536, 1747)]
 SERVE_MIX = [0.70, 0.25, 0.05]
 
 
-def serving(pmlib, O, rates, seconds: float) -> dict:
+def plain_mix(pmlib, O) -> dict:
+    """The plain per-call path on the serving mix: pm_run (the fherma_run
+    replacement) from pageable host buffers, one request at a time; median
+    seconds per point. main() runs it early: the first state made right after
+    gigabytes of device memory were freed ran its host path up to 40% slower,
+    which would flatter the comparison with the server."""
+    per_point = {}
+    for (n, w) in SERVE_POINTS:
+        q = O.spec_q(n, w)
+        L = O.limbs_of(w)
+        a, b = O.operands("uniform", n, w, q, SEED + 7)
+        c = np.empty((n, L), np.uint32)
+        with pmlib.Multiplier(n, w, O.q_limbs(q, L)) as m:
+            m.run(a, b, c)
+            ts = []
+            for _ in range(10):
+                t = time.perf_counter()
+                m.run(a, b, c)
+                ts.append(time.perf_counter() - t)
+        per_point["%d/%d" % (n, w)] = float(np.median(ts))
+    return per_point
+
+
+def serving(pmlib, O, rates, seconds: float, per_point: dict) -> dict:
     """Open-loop Poisson load on the serving mix through the native driver
     (pm_server_drive); latency from each scheduled arrival to its result in
     the caller's buffer. One server, one run per rate."""
@@ -170,21 +193,6 @@ def serving(pmlib, O, rates, seconds: float) -> dict:
         out["capacity"] = {k: cap[k] for k in ("requests", "achieved_rate", "mean_batch", "p50_ms", "p99_ms",
                                                "failed")}
         out["capacity"]["in_flight"] = 64
-    # the plain per-call path on the same mix: pm_run (the fherma_run
-    # replacement) from pageable host buffers, one request at a time
-    per_point = {}
-    for (n, w, q) in cat:
-        L = O.limbs_of(w)
-        a, b = O.operands("uniform", n, w, q, SEED + 7)
-        c = np.empty((n, L), np.uint32)
-        with pmlib.Multiplier(n, w, O.q_limbs(q, L)) as m:
-            m.run(a, b, c)
-            ts = []
-            for _ in range(10):
-                t = time.perf_counter()
-                m.run(a, b, c)
-                ts.append(time.perf_counter() - t)
-        per_point["%d/%d" % (n, w)] = float(np.median(ts))
     mean_s = sum(f * per_point["%d/%d" % p] for p, f in zip(SERVE_POINTS, SERVE_MIX))
     out["plain_per_call"] = {"median_ms": {k: round(1e3 * v, 3) for k, v in per_point.items()},
                              "mix_req_per_s": round(1.0 / mean_s, 1)}
@@ -307,6 +315,8 @@ def main():
 
     # batched: the same point 16 times, and the mixed batch of 64 uneven products
     del m, da, db, dc
+    # the serving leg's plain baseline, before the batch legs free gigabytes
+    plain_serve = plain_mix(pmlib, O) if "serve" in legs else None
     batch_same = batch_mix = None
     if "batch" in legs:
         tune = not args.no_autotune
@@ -316,7 +326,7 @@ def main():
         batch_mix = batched(pmlib, O, 64, mix, SEED + 200, 4, 50, tune)
         batch_mix["workload"] = "64 products, N log-uniform in 2^11..2^16, W uniform in [128, 2048], seed %d" % SEED
 
-    serve = serving(pmlib, O, (200, 1000, 5000), args.serve_seconds) if "serve" in legs else None
+    serve = serving(pmlib, O, (200, 1000, 5000), args.serve_seconds, plain_serve) if "serve" in legs else None
 
     cpu = cpu_baseline(q, args.cpu_cases) if "cpu" in legs else None
     out = {
