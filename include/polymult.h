@@ -50,7 +50,7 @@ int pm_run(pm_state* s, const uint32_t* a, const uint32_t* b, uint32_t* c);
 /* c = a * b with a, b, c already in device memory (N*L u32 each). Returns
  * once enqueued. With a `stream` (a hipStream_t) the call runs in order on
  * it. With NULL, consecutive calls rotate over the state's internal lanes
- * (two by default, PM_LANES=1..4; each with its own residue buffers), so one
+ * (three by default, PM_LANES=1..4; each with its own residue buffers), so one
  * call's kernels overlap the next call's; results still land in call order
  * (a call's final kernel waits for the previous call's), and pm_sync waits
  * for every lane. Calls on different streams (and pm_run, the sharded call)

@@ -53,7 +53,7 @@ struct State {
     // PM_LANES (1..kMaxLanes) sets the count. The lane streams are created
     // with the state's stream, their residues on first use.
     static constexpr int kMaxLanes = 4;
-    int lanes = 2;
+    int lanes = 3;
     hipStream_t lane_stream[kMaxLanes] = {};
     uint32_t* lane_R[kMaxLanes][2] = {};
     hipEvent_t ev_crt[kMaxLanes] = {};
