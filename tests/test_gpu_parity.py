@@ -77,6 +77,24 @@ def test_multiply_points(O, pm, N, W):
         assert np.array_equal(got, want), "%s: %d of %d coefficients differ" % (kind, _diff(got, want), N)
 
 
+def test_multiply_widest_point(O, pm):
+    """W = 4064, L = 127: the widest point. The RNS epilogue's 32-bit digit
+    sums are tightest here (4L 128^2 257 is just under 2^31), and W is a
+    multiple of 32 (the fold's whole-limb case). q is any odd 4064-bit
+    modulus with a small delta; the product mod q needs no prime."""
+    N, W = 256, 4064
+    q = (1 << W) - 1000001
+    L = O.limbs_of(W)
+    P = O.OraclePlan(N, W, q)
+    with pm.Multiplier(N, W, O.q_limbs(q, L)) as m:
+        print("multiply N=%d W=%d K=%d" % (N, W, m.K), flush=True)
+        for s, kind in enumerate(["max", "zero", "uniform", "mixed"]):
+            a, b = O.operands(kind, N, W, q, 4064 + s)
+            got = m.run(a, b)
+            want = P.multiply(a, b)
+            assert np.array_equal(got, want), "%s: %d of %d coefficients differ" % (kind, _diff(got, want), N)
+
+
 def test_multiply_schoolbook_small(O, pm):
     """Independent of the oracle's NTT: schoolbook negacyclic products."""
     for N, W in [(1, 8), (8, 33), (64, 32), (128, 256)]:
