@@ -86,6 +86,26 @@ def test_batch_closed_form_full_size(O, pm):
         assert O.unpack(dc[i].download((N, L))) == want, i
 
 
+def test_batch_widest_point(O, pm):
+    """W = 4064 (L = 127, K = 263 of the shared primes) beside narrow
+    products, in one group and as single-product groups: every product
+    equals the oracle's, which picks its own primes."""
+    qw = (1 << 4064) - 1000001  # any odd modulus with a small delta
+    prods = [(256, 4064, qw), (4096, 256, O.spec_q(4096, 256)), (2048, 4064, qw), (64, 32, O.spec_q(64, 32))]
+    ops = [O.operands(k, N, W, q, 4100 + i) for i, (k, (N, W, q)) in enumerate(zip(["max", "mixed", "uniform", "max"], prods))]
+    B = pm.Batch(prods)
+    da = [pm.DeviceBuffer.from_array(a) for a, _ in ops]
+    db = [pm.DeviceBuffer.from_array(b) for _, b in ops]
+    dc = [pm.DeviceBuffer(N * O.limbs_of(W) * 4) for N, W, _ in prods]
+    for cfg in (dict(col_tile=32, ceil_split=0, blocks_per_cu=0, group_mb=0, branches=1),
+                dict(col_tile=16, ceil_split=1, blocks_per_cu=2, group_mb=1, branches=2)):
+        print("widest point, %s" % cfg, flush=True)
+        B.configure(**cfg)
+        B.run(da, db, dc)
+        B.sync()
+        _check(O, prods, ops, dc)
+
+
 @pytest.mark.parametrize("cfg", [
     dict(col_tile=8, ceil_split=1, blocks_per_cu=2, group_mb=0, branches=1),
     dict(col_tile=16, ceil_split=0, blocks_per_cu=0, group_mb=1, branches=2),
