@@ -249,3 +249,26 @@ def test_device_calls_across_streams(O, pm):
             assert np.array_equal(dc[k].download((N, L)), want[i]), k
     s1.close()
     s2.close()
+
+
+@pytest.mark.parametrize("env", [
+    {"PM_IN_CHUNKS": "0.05,0.2", "PM_IN_TAIL": "0.03,0.01", "PM_OUT_CHUNKS": "0.1", "PM_OUT_TAIL": "0.02"},
+    {"PM_IN_CHUNKS": "500", "PM_IN_TAIL": "400", "PM_OUT_CHUNKS": "500", "PM_OUT_TAIL": "400"},
+    {"PM_HOST_THREADS": "1", "PM_H2D_STREAMS": "1"},
+    {"PM_HOST_THREADS": "3", "PM_IN_CHUNKS": "1,0.3", "PM_OUT_CHUNKS": "0.7,2"},
+])
+def test_host_pipeline_plans(O, pm, env, monkeypatch):
+    """pm_run (the fherma_run path) under other chunk plans, host-thread
+    counts and one H2D stream: dozens of tiny chunks, one chunk larger than
+    the operand, tails larger than N. The plans are read when a state is
+    created."""
+    for k, v in env.items():
+        monkeypatch.setenv(k, v)
+    print("pipeline plan %s" % env, flush=True)
+    for N, W in [(32768, 868), (4096, 256), (1, 8), (64, 32)]:
+        q, L, m, P = _setup(O, pm, N, W)
+        with m:
+            for s, kind in enumerate(["mixed", "max"]):
+                a, b = O.operands(kind, N, W, q, 1300 + s + N)
+                got = m.run(a, b)
+                assert np.array_equal(got, P.multiply(a, b)), (N, W, kind)
