@@ -136,3 +136,30 @@ def test_serve_concurrent_submitters(O, pm):
         assert not any(th.is_alive() for th in threads)
         assert not errors, errors
         assert S.stats()["completed"] == 24
+
+
+@pytest.mark.parametrize("config", [
+    dict(max_inflight=1, max_batch=2, host_threads=1, max_batch_bytes=1 << 20),
+    dict(max_inflight=6, max_batch=64, host_threads=4, col_tile=8),
+])
+def test_serve_other_configs(O, pm, config):
+    """One slot with tiny batches and one host thread, or six slots and
+    narrow column tiles: every request submitted at once still completes,
+    each equal to the plain per-call path."""
+    cat = _catalog(O, [(8192, 438), (1024, 256), (32768, 868)])
+    reqs = []
+    for i in range(10):
+        p = i % len(cat)
+        N, W, q = cat[p]
+        reqs.append((p, *O.operands(["mixed", "max", "uniform"][i % 3], N, W, q, 500 + i)))
+    print("server config %s" % config, flush=True)
+    with pm.Server(cat, **config) as S:
+        tickets = [S.submit(p, a, b) for p, a, b in reqs]
+        got = [S.wait(t)[0] for t in tickets]
+        st = S.stats()
+        assert st["completed"] == len(reqs) and st["failed"] == 0
+        if config["max_batch"] == 2:
+            assert st["batches"] >= len(reqs) // 2
+    for i, (p, a, b) in enumerate(reqs):
+        N, W, q = cat[p]
+        assert np.array_equal(got[i], O.OraclePlan(N, W, q).multiply(a, b)), "request %d" % i
