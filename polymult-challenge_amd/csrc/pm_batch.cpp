@@ -346,11 +346,15 @@ struct Batch {
         drop_graph();
     }
     void set_operands(const uint32_t* const* a, const uint32_t* const* b, uint32_t* const* c) {
-        bool changed = false;
+        // every pointer is checked before any is taken, so a refused call
+        // leaves the batch's pointers (host and device copies) in step
         for (size_t i = 0; i < run.prods.size(); ++i) {
             if (!a[i] || !b[i] || !c[i]) throw std::runtime_error("null operand pointer");
             if ((uintptr_t)a[i] % 16 || (uintptr_t)b[i] % 16)
                 throw std::runtime_error("batch operands must be 16-byte aligned");
+        }
+        bool changed = false;
+        for (size_t i = 0; i < run.prods.size(); ++i) {
             BatchProduct& P = run.prods[i];
             if (P.a != a[i] || P.b != b[i] || P.c != c[i]) {
                 P.a = a[i];
@@ -508,6 +512,7 @@ int pm_batch_product_info(const pm_batch* b, uint32_t i, pm_info* out) {
 int pm_batch_run(pm_batch* b, const uint32_t* const* d_a, const uint32_t* const* d_b, uint32_t* const* d_c,
                  void* stream) {
     return guarded([&] {
+        if (!b || !d_a || !d_b || !d_c) throw std::runtime_error("null argument");
         pm::Batch& B = *b->b;
         B.set_operands(d_a, d_b, d_c);
         hipStream_t st = stream ? static_cast<hipStream_t>(stream) : B.stream;
@@ -531,6 +536,7 @@ int pm_batch_sync(pm_batch* b) {
 int pm_batch_time(pm_batch* b, const uint32_t* const* d_a, const uint32_t* const* d_b, uint32_t* const* d_c,
                   int iters, float* ms) {
     return guarded([&] {
+        if (!b || !d_a || !d_b || !d_c || !ms) throw std::runtime_error("null argument");
         pm::Batch& B = *b->b;
         B.set_operands(d_a, d_b, d_c);
         *ms = B.time_ms(iters);
@@ -557,6 +563,7 @@ int pm_batch_configure(pm_batch* b, int col_tile, int ceil_split, int blocks_per
 int pm_batch_autotune(pm_batch* b, const uint32_t* const* d_a, const uint32_t* const* d_b, uint32_t* const* d_c,
                       int iters, int* best) {
     return guarded([&] {
+        if (!b || !d_a || !d_b || !d_c || !best) throw std::runtime_error("null argument");
         pm::Batch& B = *b->b;
         B.set_operands(d_a, d_b, d_c);
         pm::BatchConfig bestc = B.run.cfg;
