@@ -179,7 +179,7 @@ def test_errors_like_reference(O, pm):
 
 
 def test_device_calls_in_flight(O, pm):
-    """Default-stream device calls overlap on two lanes: each result is exact,
+    """Default-stream device calls overlap on the lanes: each result is exact,
     and calls into one output buffer leave the last call's result."""
     N, W = 32768, 868
     q = O.spec_q(N, W)
@@ -249,6 +249,50 @@ def test_device_calls_across_streams(O, pm):
             assert np.array_equal(dc[k].download((N, L)), want[i]), k
     s1.close()
     s2.close()
+
+
+def test_distinct_states_on_threads(O, pm):
+    """Four states, each created, used and freed on its own host thread at
+    once (the ctypes calls release the GIL): host-buffer and device calls on
+    every state are exact, as the header promises for distinct states."""
+    import threading
+
+    points = [(4096, 256), (2048, 868), (8192, 100), (1024, 1747)]
+    jobs = []
+    for i, (N, W) in enumerate(points):
+        q = O.spec_q(N, W)
+        ops = [O.operands(k, N, W, q, 1300 + 10 * i + j) for j, k in enumerate(["uniform", "mixed"])]
+        jobs.append((N, W, q, ops, [O.OraclePlan(N, W, q).multiply(a, b) for a, b in ops]))
+    errors = []
+    start = threading.Barrier(len(jobs))
+
+    def work(N, W, q, ops, want):
+        try:
+            L = O.limbs_of(W)
+            start.wait(60)
+            with pm.Multiplier(N, W, O.q_limbs(q, L)) as m:
+                da = [pm.DeviceBuffer.from_array(a) for a, _ in ops]
+                db = [pm.DeviceBuffer.from_array(b) for _, b in ops]
+                dc = [pm.DeviceBuffer(N * L * 4) for _ in ops]
+                for rep in range(4):
+                    for j in range(len(ops)):
+                        if not np.array_equal(m.run(*ops[j]), want[j]):
+                            errors.append("host N=%d W=%d rep %d op %d" % (N, W, rep, j))
+                        m.run_device(da[j], db[j], dc[j])
+                    m.sync()
+                    for j in range(len(ops)):
+                        if not np.array_equal(dc[j].download((N, L)), want[j]):
+                            errors.append("device N=%d W=%d rep %d op %d" % (N, W, rep, j))
+        except Exception as e:  # reported below, on the test's thread
+            errors.append("N=%d W=%d: %r" % (N, W, e))
+
+    threads = [threading.Thread(target=work, args=job) for job in jobs]
+    for t in threads:
+        t.start()
+    for t in threads:
+        t.join(100)
+    assert not any(t.is_alive() for t in threads), "a worker did not finish"
+    assert not errors, errors
 
 
 @pytest.mark.parametrize("env", [
