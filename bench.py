@@ -338,6 +338,9 @@ def main():
                    "operands": "mixed per coefficient: 1/2 uniform [0,q), 1/4 q-1, 1/4 < 2^32", "seed": SEED},
         "bit_exact": correct,
         "pct_hbm_peak": round(100 * pcie_bytes / (ms_step * 1e-3) / (HBM_PEAK_GBS * 1e9), 3),
+        # M = K (3 (N/2) log2 N + 2N) modmuls per product, over the throughput, against the VALU peak
+        "modmuls_per_product": sum(mm.values()),
+        "pct_valu_modmul_peak": round(100 * sum(mm.values()) / (ms_step * 1e-3) / PEAK_MODMUL_PER_S, 2),
         "serial_multiply_ms": round(ms[0], 4),
         "roofline": roofline, "kernels": per_kernel, "end_to_end": end_to_end, "prime_shard": shard,
         "batched_same_point": batch_same, "batched_mixed": batch_mix, "serving": serve, "cpu_baseline": cpu,
