@@ -62,7 +62,9 @@ PM_HD void ct_butterfly(uint32_t& x0, uint32_t& x1, uint32_t w, uint32_t wq, uin
 PM_HD void gs_butterfly(uint32_t& x0, uint32_t& x1, uint32_t m, uint32_t mq, uint32_t p) {
     const uint32_t u = x0, v = x1;
     x0 = add_mod(u, v, p);
-    x1 = shoup_mul(sub_mod(v, u, p), m, mq, p);
+    // v - u + p lies in [1, 2p), below 2^32: Shoup's product takes it as it
+    // is, so the difference needs no reduction of its own.
+    x1 = shoup_mul(v + (p - u), m, mq, p);
 }
 
 }  // namespace pm
