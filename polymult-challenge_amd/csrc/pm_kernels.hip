@@ -70,9 +70,11 @@ __device__ __forceinline__ void fwd_pass1(u32 (&x)[NA][Line<NB>::E], const uint2
 #pragma unroll
     for (int st = 0; st < R1; ++st) {
         const int half = E >> (st + 1);
+        // the stage's first entry as a pointer, so b becomes an immediate offset
+        const uint2* ts = tab + ((1u << (s0 + st)) + (line << st));
 #pragma unroll
         for (int b = 0; b < (1 << st); ++b) {
-            const uint2 w = tab[(1u << (s0 + st)) + (line << st) + b];
+            const uint2 w = ts[b];
 #pragma unroll
             for (int a = 0; a < NA; ++a)
 #pragma unroll
@@ -91,9 +93,10 @@ __device__ __forceinline__ void fwd_pass2(u32 (&x)[NA][Line<NB>::E], const uint2
         const int ls = R1 + st;
         const int half = 1 << (R2 - 1 - st);
         const int nblk = E / (2 * half);
+        const uint2* ts = tab + ((1u << (s0 + ls)) + (line << ls) + (u32)(t * nblk));
 #pragma unroll
         for (int b = 0; b < nblk; ++b) {
-            const uint2 w = tab[(1u << (s0 + ls)) + (line << ls) + (u32)(t * nblk + b)];
+            const uint2 w = ts[b];
 #pragma unroll
             for (int a = 0; a < NA; ++a)
 #pragma unroll
@@ -112,10 +115,12 @@ __device__ __forceinline__ void inv_pass2(u32 (&x)[Line<NB>::E], const uint2* __
         const int ls = R1 + st;
         const int half = 1 << (R2 - 1 - st);
         const int nblk = E / (2 * half);
+        // entry (2^(s0+ls+1) - 1 - i) for i = (line << ls) + t nblk + b: the
+        // last block's entry as a pointer, b counted back from it
+        const uint2* ts = tab + ((2u << (s0 + ls)) - 1u - ((line << ls) + (u32)(t * nblk + nblk - 1)));
 #pragma unroll
         for (int b = 0; b < nblk; ++b) {
-            const u32 i = (line << ls) + (u32)(t * nblk + b);
-            const uint2 m = tab[(2u << (s0 + ls)) - 1u - i];
+            const uint2 m = ts[nblk - 1 - b];
 #pragma unroll
             for (int qq = 0; qq < half; ++qq) gs_butterfly(x[2 * b * half + qq], x[2 * b * half + qq + half], m.x, m.y, p);
         }
@@ -129,10 +134,10 @@ __device__ __forceinline__ void inv_pass1(u32 (&x)[Line<NB>::E], const uint2* __
 #pragma unroll
     for (int st = R1 - 1; st >= 0; --st) {
         const int half = E >> (st + 1);
+        const uint2* ts = tab + ((2u << (s0 + st)) - 1u - ((line << st) + (u32)((1 << st) - 1)));
 #pragma unroll
         for (int b = 0; b < (1 << st); ++b) {
-            const u32 i = (line << st) + (u32)b;
-            const uint2 m = tab[(2u << (s0 + st)) - 1u - i];
+            const uint2 m = ts[(1 << st) - 1 - b];
 #pragma unroll
             for (int qq = 0; qq < half; ++qq) gs_butterfly(x[2 * b * half + qq], x[2 * b * half + qq + half], m.x, m.y, p);
         }
@@ -157,8 +162,17 @@ __device__ __forceinline__ void col_tile(const DevPlan& D, u32* __restrict__ R, 
     const int c = tid % tc, t = tid / tc;
     const u32 p = D.p[j];
     const uint2* tab = D.tw + (size_t)j * D.tw_stride;
-    const size_t ncols = (size_t)1 << D.w2;
-    u32* col = R + (size_t)j * D.N + col0 + c;
+    // The tile's base is the same for the whole block; each access adds a
+    // 32-bit byte offset to it (the prime's vector is below 2^32 bytes), so
+    // the loads and stores take a scalar base and no 64-bit address math.
+    // Row r of the tile sits r 2^w2 words on; the offsets are split into the
+    // thread's part and a uniform step per k.
+    unsigned char* base = reinterpret_cast<unsigned char*>(R + (size_t)j * D.N + col0);
+    const u32 w2 = D.w2;
+    const u32 strided0 = (((u32)t << w2) + (u32)c) << 2;      // rows t + k T
+    const u32 blocked0 = (((u32)(t * E) << w2) + (u32)c) << 2;  // rows t E + k
+    auto strided = [&](int k) -> u32& { return *reinterpret_cast<u32*>(base + strided0 + ((u32)(k * T) << (w2 + 2))); };
+    auto blocked = [&](int k) -> u32& { return *reinterpret_cast<u32*>(base + blocked0 + ((u32)k << (w2 + 2))); };
     u32 x[1][E];
     uint2* ltw = reinterpret_cast<uint2*>(lds + (1 << NB) * tc);
     auto stage_twiddles = [&] {
@@ -168,7 +182,7 @@ __device__ __forceinline__ void col_tile(const DevPlan& D, u32* __restrict__ R, 
     if (!INV) {
         if (on) {
 #pragma unroll
-            for (int k = 0; k < E; ++k) x[0][k] = col[(size_t)(t + k * T) * ncols];
+            for (int k = 0; k < E; ++k) x[0][k] = strided(k);
         }
         if constexpr (R2 > 0) stage_twiddles();  // read after the transpose barrier
         if (on) fwd_pass1<NB, 1>(x, tab, 0, 0, p);
@@ -183,17 +197,17 @@ __device__ __forceinline__ void col_tile(const DevPlan& D, u32* __restrict__ R, 
                 for (int k = 0; k < E; ++k) x[0][k] = lds[(t * E + k) * tc + c];
                 fwd_pass2<NB, 1>(x, ltw, 0, 0, t, p);
 #pragma unroll
-                for (int k = 0; k < E; ++k) col[(size_t)(t * E + k) * ncols] = x[0][k];
+                for (int k = 0; k < E; ++k) blocked(k) = x[0][k];
             }
         } else if (on) {
 #pragma unroll
-            for (int k = 0; k < E; ++k) col[(size_t)(t + k * T) * ncols] = x[0][k];
+            for (int k = 0; k < E; ++k) strided(k) = x[0][k];
         }
     } else {
         if constexpr (R2 > 0) {
             if (on) {
 #pragma unroll
-                for (int k = 0; k < E; ++k) x[0][k] = col[(size_t)(t * E + k) * ncols];
+                for (int k = 0; k < E; ++k) x[0][k] = blocked(k);
             }
             stage_twiddles();
             __syncthreads();
@@ -209,12 +223,12 @@ __device__ __forceinline__ void col_tile(const DevPlan& D, u32* __restrict__ R, 
             }
         } else if (on) {
 #pragma unroll
-            for (int k = 0; k < E; ++k) x[0][k] = col[(size_t)(t + k * T) * ncols];
+            for (int k = 0; k < E; ++k) x[0][k] = strided(k);
         }
         if (on) {
             inv_pass1<NB>(x[0], tab, 0, 0, p);
 #pragma unroll
-            for (int k = 0; k < E; ++k) col[(size_t)(t + k * T) * ncols] = x[0][k];
+            for (int k = 0; k < E; ++k) strided(k) = x[0][k];
         }
     }
 }
