@@ -153,9 +153,12 @@ __device__ __forceinline__ void inv_pass1(u32 (&x)[Line<NB>::E], const uint2* __
 // stage. Threads sweep columns fastest, so every global access of a wave
 // reads or writes whole row segments.
 // ---------------------------------------------------------------------------
-template <int NB, bool INV>
-__device__ __forceinline__ void col_tile(const DevPlan& D, u32* __restrict__ R, int j, int col0, int tc, u32* lds,
+// TC > 0 fixes the tile width at compile time (the plain path's 32 columns):
+// the LDS offsets of the unrolled accesses become immediates.
+template <int NB, bool INV, int TC = 0>
+__device__ __forceinline__ void col_tile(const DevPlan& D, u32* __restrict__ R, int j, int col0, int tc_in, u32* lds,
                                          int tid) {
+    const int tc = TC > 0 ? TC : tc_in;
     using Ln = Line<NB>;
     constexpr int E = Ln::E, T = Ln::T, R2 = Ln::R2;
     const bool on = tid < tc * T;
@@ -233,10 +236,10 @@ __device__ __forceinline__ void col_tile(const DevPlan& D, u32* __restrict__ R, 
     }
 }
 
-template <int NB, bool INV>
+template <int NB, bool INV, int TC>
 __global__ void __launch_bounds__(1024) k_col(DevPlan D, u32* __restrict__ R0, u32* __restrict__ R1, int tc) {
     extern __shared__ u32 lds[];
-    col_tile<NB, INV>(D, blockIdx.z ? R1 : R0, (int)blockIdx.y, (int)blockIdx.x * tc, tc, lds, (int)threadIdx.x);
+    col_tile<NB, INV, TC>(D, blockIdx.z ? R1 : R0, (int)blockIdx.y, (int)blockIdx.x * tc, tc, lds, (int)threadIdx.x);
 }
 
 // ---------------------------------------------------------------------------
@@ -245,12 +248,14 @@ __global__ void __launch_bounds__(1024) k_col(DevPlan D, u32* __restrict__ R0, u
 // of the rows, the other rpb/2 are their mirrors nrows - 1 - r. The inverse
 // stages of row r read the twiddles the forward stages of row nrows - 1 - r
 // read (the mirrored entries), so a pair shares its twiddle lines in cache
-// and the table streams from HBM once instead of twice. LDS rows are padded
-// by 16 words and XOR-swizzled inside each 16-word group so that both the
-// strided pass-1 writes and the contiguous pass-2 reads spread over the
-// banks. LDS: rpb (2^NB + 16) words (one row buffer per row).
+// and the table streams from HBM once instead of twice. Row position e sits
+// at e + e/16 in its LDS row (one pad word after every 16) and rows are
+// 2^NB + 16 words apart, so both the strided pass-1 accesses (t + k T) and
+// the contiguous pass-2 accesses (16 t + k, stride 17 across the threads)
+// spread over the banks. The padded index is the thread's base plus a
+// constant for every k, so each access takes an immediate offset.
+// LDS: rpb (2^NB + 16) words (one row buffer per row).
 // ---------------------------------------------------------------------------
-__device__ __forceinline__ int swz(int e) { return e ^ ((e >> 4) & 15); }
 
 template <int NB, int MODE>
 __device__ __forceinline__ void row_tile(const DevPlan& D, u32* __restrict__ RB, const u32* __restrict__ RA, int j,
@@ -272,6 +277,11 @@ __device__ __forceinline__ void row_tile(const DevPlan& D, u32* __restrict__ RB,
     u32* rb = RB + off;
     const int s0 = D.w1;
     u32* lb = lds + r * RS;
+    // padded LDS index of position t + k T (t < T, T a power of two up to 16:
+    // (t + k T) / 16 = k T / 16) and of position 16 t + k (k < 16)
+    static_assert(R2 == 0 || (E == 16 && T <= 16), "pad mapping assumes 16 values per thread");
+    auto ps = [&](int k) { return t + k * T + ((k * T) >> 4); };
+    auto pc = [&](int k) { return t * (E + 1) + k; };
 
     if constexpr (MODE == ROW_FUSED) {
         const u32* ra = RA + off;
@@ -293,12 +303,12 @@ __device__ __forceinline__ void row_tile(const DevPlan& D, u32* __restrict__ RB,
                 if (a) __syncthreads();  // every read of a's transpose is done
                 if (on) {
 #pragma unroll
-                    for (int k = 0; k < E; ++k) lb[swz(t + k * T)] = x[a][k];
+                    for (int k = 0; k < E; ++k) lb[ps(k)] = x[a][k];
                 }
                 __syncthreads();
                 if (on) {
 #pragma unroll
-                    for (int k = 0; k < E; ++k) x[a][k] = lb[swz(t * E + k)];
+                    for (int k = 0; k < E; ++k) x[a][k] = lb[pc(k)];
                 }
             }
             if (on) fwd_pass2<NB, 2>(x, tab, s0, row, t, p);
@@ -313,12 +323,12 @@ __device__ __forceinline__ void row_tile(const DevPlan& D, u32* __restrict__ RB,
             __syncthreads();
             if (on) {
 #pragma unroll
-                for (int k = 0; k < E; ++k) lb[swz(t * E + k)] = y[k];
+                for (int k = 0; k < E; ++k) lb[pc(k)] = y[k];
             }
             __syncthreads();
             if (on) {
 #pragma unroll
-                for (int k = 0; k < E; ++k) y[k] = lb[swz(t + k * T)];
+                for (int k = 0; k < E; ++k) y[k] = lb[ps(k)];
             }
         }
         if (on) {
@@ -336,12 +346,12 @@ __device__ __forceinline__ void row_tile(const DevPlan& D, u32* __restrict__ RB,
         if constexpr (R2 > 0) {
             if (on) {
 #pragma unroll
-                for (int k = 0; k < E; ++k) lb[swz(t + k * T)] = x[0][k];
+                for (int k = 0; k < E; ++k) lb[ps(k)] = x[0][k];
             }
             __syncthreads();
             if (on) {
 #pragma unroll
-                for (int k = 0; k < E; ++k) x[0][k] = lb[swz(t * E + k)];
+                for (int k = 0; k < E; ++k) x[0][k] = lb[pc(k)];
                 fwd_pass2<NB, 1>(x, tab, s0, row, t, p);
 #pragma unroll
                 for (int k = 0; k < E; ++k) rb[t * E + k] = x[0][k];
@@ -358,12 +368,12 @@ __device__ __forceinline__ void row_tile(const DevPlan& D, u32* __restrict__ RB,
                 for (int k = 0; k < E; ++k) y[k] = rb[t * E + k];
                 inv_pass2<NB>(y, tab, s0, row, t, p);
 #pragma unroll
-                for (int k = 0; k < E; ++k) lb[swz(t * E + k)] = y[k];
+                for (int k = 0; k < E; ++k) lb[pc(k)] = y[k];
             }
             __syncthreads();
             if (on) {
 #pragma unroll
-                for (int k = 0; k < E; ++k) y[k] = lb[swz(t + k * T)];
+                for (int k = 0; k < E; ++k) y[k] = lb[ps(k)];
             }
         } else if (on) {
 #pragma unroll
@@ -858,8 +868,13 @@ template <bool INV>
 hipError_t col_dispatch(int nb, dim3 grid, dim3 block, size_t lds, hipStream_t s, const DevPlan& D, u32* R0, u32* R1,
                         int tc) {
     switch (nb) {
-#define PM_COL(n) \
-    case n: hipLaunchKernelGGL((k_col<n, INV>), grid, block, lds, s, D, R0, R1, tc); break;
+#define PM_COL(n)                                                                         \
+    case n:                                                                               \
+        if (tc == 32)                                                                     \
+            hipLaunchKernelGGL((k_col<n, INV, 32>), grid, block, lds, s, D, R0, R1, tc);  \
+        else                                                                              \
+            hipLaunchKernelGGL((k_col<n, INV, 0>), grid, block, lds, s, D, R0, R1, tc);   \
+        break;
         PM_COL(1) PM_COL(2) PM_COL(3) PM_COL(4) PM_COL(5) PM_COL(6) PM_COL(7) PM_COL(8)
 #undef PM_COL
         default: return hipErrorInvalidValue;
