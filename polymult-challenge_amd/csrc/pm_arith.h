@@ -57,6 +57,16 @@ PM_HD void ct_butterfly(uint32_t& x0, uint32_t& x1, uint32_t w, uint32_t wq, uin
     x1 = sub_mod(u, v, p);
 }
 
+// The same butterfly for outputs that the next stage only multiplies by a
+// twiddle: Shoup's product takes any 32-bit input, so x0 = u + v in [0, 2p)
+// and x1 = u - v + p in [1, 2p) stay unreduced (x0 must come in below p).
+PM_HD void ct_butterfly_lazy(uint32_t& x0, uint32_t& x1, uint32_t w, uint32_t wq, uint32_t p) {
+    const uint32_t v = shoup_mul(x1, w, wq, p);
+    const uint32_t u = x0;
+    x0 = u + v;
+    x1 = u + (p - v);
+}
+
 // Gentleman-Sande with the mirrored twiddle m = -w^-1:
 // (x0, x1) -> (x0 + x1, (x0 - x1) w^-1) = (x0 + x1, (x1 - x0) m).
 PM_HD void gs_butterfly(uint32_t& x0, uint32_t& x1, uint32_t m, uint32_t mq, uint32_t p) {

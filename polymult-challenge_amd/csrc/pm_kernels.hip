@@ -70,6 +70,7 @@ __device__ __forceinline__ void fwd_pass1(u32 (&x)[NA][Line<NB>::E], const uint2
 #pragma unroll
     for (int st = 0; st < R1; ++st) {
         const int half = E >> (st + 1);
+        const bool LAZY = st < R1 - 1;  // the pass's last stage leaves every output reduced
         // the stage's first entry as a pointer, so b becomes an immediate offset
         const uint2* ts = tab + ((1u << (s0 + st)) + (line << st));
 #pragma unroll
@@ -78,8 +79,13 @@ __device__ __forceinline__ void fwd_pass1(u32 (&x)[NA][Line<NB>::E], const uint2
 #pragma unroll
             for (int a = 0; a < NA; ++a)
 #pragma unroll
-                for (int qq = 0; qq < half; ++qq)
-                    ct_butterfly(x[a][2 * b * half + qq], x[a][2 * b * half + qq + half], w.x, w.y, p);
+                for (int qq = 0; qq < half; ++qq) {
+                    // outputs at qq >= half/2 are the next stage's twiddled inputs
+                    if (LAZY && qq >= half / 2)
+                        ct_butterfly_lazy(x[a][2 * b * half + qq], x[a][2 * b * half + qq + half], w.x, w.y, p);
+                    else
+                        ct_butterfly(x[a][2 * b * half + qq], x[a][2 * b * half + qq + half], w.x, w.y, p);
+                }
         }
     }
 }
@@ -92,6 +98,7 @@ __device__ __forceinline__ void fwd_pass2(u32 (&x)[NA][Line<NB>::E], const uint2
     for (int st = 0; st < R2; ++st) {
         const int ls = R1 + st;
         const int half = 1 << (R2 - 1 - st);
+        const bool LAZY = st < R2 - 1;
         const int nblk = E / (2 * half);
         const uint2* ts = tab + ((1u << (s0 + ls)) + (line << ls) + (u32)(t * nblk));
 #pragma unroll
@@ -100,8 +107,13 @@ __device__ __forceinline__ void fwd_pass2(u32 (&x)[NA][Line<NB>::E], const uint2
 #pragma unroll
             for (int a = 0; a < NA; ++a)
 #pragma unroll
-                for (int qq = 0; qq < half; ++qq)
-                    ct_butterfly(x[a][2 * b * half + qq], x[a][2 * b * half + qq + half], w.x, w.y, p);
+                for (int qq = 0; qq < half; ++qq) {
+                    // outputs at qq >= half/2 are the next stage's twiddled inputs
+                    if (LAZY && qq >= half / 2)
+                        ct_butterfly_lazy(x[a][2 * b * half + qq], x[a][2 * b * half + qq + half], w.x, w.y, p);
+                    else
+                        ct_butterfly(x[a][2 * b * half + qq], x[a][2 * b * half + qq + half], w.x, w.y, p);
+                }
         }
     }
 }
