@@ -178,10 +178,10 @@ __device__ __forceinline__ void col_tile(const DevPlan& D, u32* __restrict__ R, 
     const u32 p = D.p[j];
     const uint2* tab = D.tw + (size_t)j * D.tw_stride;
     // The tile's base is the same for the whole block; each access adds a
-    // 32-bit byte offset to it (the prime's vector is below 2^32 bytes), so
-    // the loads and stores take a scalar base and no 64-bit address math.
-    // Row r of the tile sits r 2^w2 words on; the offsets are split into the
-    // thread's part and a uniform step per k.
+    // 32-bit byte offset to it (the prime's vector is below 2^32 bytes), one
+    // 64-bit add per access instead of a sign extension, a 64-bit shift and
+    // an add. Row r of the tile sits r 2^w2 words on; the offsets are split
+    // into the thread's part and a uniform step per k.
     unsigned char* base = reinterpret_cast<unsigned char*>(R + (size_t)j * D.N + col0);
     const u32 w2 = D.w2;
     const u32 strided0 = (((u32)t << w2) + (u32)c) << 2;      // rows t + k T
