@@ -54,6 +54,29 @@ def test_stage_ntt(O, pm, N, W):
     assert np.array_equal(inv, want)
 
 
+@pytest.mark.parametrize("N,W", [(8192, 256), (65536, 1747)])
+def test_stage_ntt_extremes(O, pm, N, W):
+    """Residues at the ends of [0, p): the forward butterflies that leave
+    u + v in [0, 2p) and u - v + p in [1, 2p) unreduced, and the inverse
+    difference v - u + p, meet their largest and smallest values here."""
+    q, L, m, P = _setup(O, pm, N, W)
+    print("stage ntt extremes N=%d W=%d K=%d" % (N, W, m.K), flush=True)
+    p = P.primes.astype(np.uint64)[:, None]
+    rng = np.random.default_rng(7 * N + W)
+    spikes = np.zeros((m.K, N), np.uint64)
+    spikes[:, rng.choice(N, 64, replace=False)] = 1
+    cases = {
+        "all p-1": np.broadcast_to(p - 1, (m.K, N)),
+        "p-1 spikes on zeros": spikes * (p - 1),
+        "0, 1, p-2, p-1": np.choose(rng.integers(0, 4, size=(m.K, N)), [np.zeros_like(p), np.ones_like(p), p - 2, p - 1]),
+    }
+    for name, R64 in cases.items():
+        R = np.ascontiguousarray(R64, dtype=np.uint64).astype(np.uint32)
+        assert np.array_equal(m.stage_ntt_fwd(R), P.ntt_fwd(R)), "forward, " + name
+        want = (P.ntt_inv(R).astype(np.uint64) * np.uint64(N) % p).astype(np.uint32)
+        assert np.array_equal(m.stage_ntt_inv(R), want), "inverse, " + name
+
+
 @pytest.mark.parametrize("N,W", [(1, 8), (64, 32), (1024, 512), (2048, 868), (512, 4000), (65536, 1747)])
 def test_stage_crt(O, pm, N, W):
     q, L, m, P = _setup(O, pm, N, W)
