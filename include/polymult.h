@@ -63,15 +63,21 @@ int pm_run_device(pm_state* s, const uint32_t* d_a, const uint32_t* d_b, uint32_
 /* Releases everything. Replaces fherma_free (fherma.h:48; solve.cu:1531-1535). */
 void pm_free(pm_state* s);
 
+/* The calling thread's last error message, worded as the reference's
+ * exceptions (solve.cu:873-913, 1482-1483), which the C ABI cannot throw. */
 const char* pm_last_error(void);
+/* The plan's sizes. No counterpart in the reference (its state is opaque). */
 int pm_query(const pm_state* s, pm_info* out);
 
 /* Host-only point check (no GPU needed): validates like pm_init and reports
- * K, delta and up to `cap` primes. */
+ * K, delta and up to `cap` primes, as the reference's precompute
+ * (solve.cu:869-1020) derives them. */
 int pm_point_info(uint32_t N, uint32_t W, uint32_t L, const uint32_t* q, size_t q_limbs, uint32_t* K,
                   uint64_t* delta, uint32_t* primes, size_t cap);
 
-/* ---- device buffers and timing, for callers without another GPU runtime ---- */
+/* ---- device buffers and timing, for callers without another GPU runtime ----
+ * No counterpart in the reference, whose only entry points take host
+ * buffers; these serve the device-resident calls above. */
 int pm_dev_alloc(size_t bytes, void** out);
 int pm_dev_free(void* p);
 int pm_copy_to_device(void* dst, const void* src, size_t bytes);
