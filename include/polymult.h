@@ -87,14 +87,16 @@ int pm_copy_to_host(void* dst, const void* src, size_t bytes);
 int pm_stream_create(void** out);
 int pm_stream_sync(void* stream);
 int pm_stream_destroy(void* stream);
-/* Waits for every device call of the state (every lane). */
+/* Waits for every device call of the state (every lane). No counterpart in
+ * the reference (its fherma_run returns when c is on the host). */
 int pm_sync(pm_state* s);
 
 /* Average milliseconds, from HIP events on the state's stream, of `iters`
  * launches enqueued back to back (after one warm launch): ms[1] RNS, ms[2]
  * column forward, ms[3] fused rows, ms[4] column inverse, ms[5] CRT (each
  * kernel repeated alone, on the residues the chain leaves), then ms[0] the
- * whole multiply. d_c holds a * b afterwards. */
+ * whole multiply. d_c holds a * b afterwards. No counterpart in the
+ * reference: a timing diagnostic of the device-resident form. */
 int pm_time_device(pm_state* s, const uint32_t* d_a, const uint32_t* d_b, uint32_t* d_c, int iters, float* ms);
 
 /* PCIe and host-copy rates (GB/s, best of `reps`) for transfers of `bytes`:
@@ -124,7 +126,8 @@ int pm_time_shard(pm_state* s, const uint32_t* d_a, const uint32_t* d_b, uint32_
 
 /* HBM copy rate (GB/s, read + write) of a plain device copy kernel over
  * `bytes` (a multiple of 16), averaged over `iters` launches. Diagnostic: the
- * reference point for the HBM traffic counters and for achievable bandwidth. */
+ * reference point for the HBM traffic counters and for achievable bandwidth.
+ * No counterpart in the reference. */
 int pm_probe_copy(size_t bytes, int iters, double* gbps);
 
 /* ---- stage entry points, for stage-by-stage parity tests ----
@@ -133,7 +136,12 @@ int pm_probe_copy(size_t bytes, int iters, double* gbps);
  *   ntt_fwd: R <- forward negacyclic NTT of each prime row (in place),
  *            bit-reversed output order
  *   ntt_inv: R <- N * inverse NTT (unscaled)
- *   crt:     c[N][L] from canonical residues X[K][N] of the integer product */
+ *   crt:     c[N][L] from canonical residues X[K][N] of the integer product
+ * The reference exposes no such entry points; each stage computes what one
+ * of its kernels computes: rns = k_decompose (solve.cu:147-202), ntt_fwd =
+ * k_col<false> then the forward rows (solve.cu:254-355), ntt_inv = the inverse
+ * rows then k_col<true> (solve.cu:254-284, 357-378), crt = k_crt
+ * (solve.cu:380-501). */
 int pm_stage_rns(pm_state* s, const uint32_t* d_a, uint32_t* d_R);
 int pm_stage_ntt_fwd(pm_state* s, uint32_t* d_R);
 int pm_stage_ntt_inv(pm_state* s, uint32_t* d_R);
