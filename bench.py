@@ -8,7 +8,7 @@ through the C ABI (pm_run_device). `value` is multiplies per second over the
 timed steps. The PCIe-inclusive rate of the host-buffer call (pm_run, the
 fherma_run replacement) is reported beside it, never as `value`.
 
-    python bench.py [--gpus 1] [--steps 200] [--warmup 20] [--no-cpu] [--serve-seconds 10] [--no-serve]
+    python bench.py [--gpus 1] [--steps 2000] [--warmup 200] [--no-cpu] [--serve-seconds 10] [--no-serve]
                     [--legs plain,shard,e2e,batch,serve,cpu] [--copy-probe BYTES]
 
 Prints one JSON line.
