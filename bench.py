@@ -309,8 +309,15 @@ def main():
             m.run(a, b, c_host)
             e2e.append(time.perf_counter() - t)
         e2e_med = float(np.median(e2e))
+        # the same against the pinned copy rates measured here: one product's
+        # floor is its inputs in (8 N L bytes) then its result out (4 N L)
+        tr = pmlib.probe_transfers(64 << 20, 16, 3)
+        floor_s = 8 * N * L / (tr["h2d_pinned"] * 1e9) + 4 * N * L / (tr["d2h_pinned"] * 1e9)
         end_to_end = {"mults_per_s": round(1.0 / e2e_med, 2), "median_ms": round(1e3 * e2e_med, 3),
                       "pct_pcie_peak": round(100 * pcie_bytes / e2e_med / (PCIE_PEAK_GBS * 1e9), 1),
+                      "measured_pinned_GBps": {"h2d": tr["h2d_pinned"], "d2h": tr["d2h_pinned"]},
+                      "floor_ms_at_measured": round(1e3 * floor_s, 3),
+                      "pct_of_measured_floor": round(100 * floor_s / e2e_med, 1),
                       "bit_exact": bool(np.array_equal(c_host, c_dev))}
 
     # batched: the same point 16 times, and the mixed batch of 64 uneven products
