@@ -8,8 +8,12 @@ through the C ABI (pm_run_device). `value` is multiplies per second over the
 timed steps. The PCIe-inclusive rate of the host-buffer call (pm_run, the
 fherma_run replacement) is reported beside it, never as `value`.
 
-    python bench.py [--gpus 1] [--steps 2000] [--warmup 200] [--no-cpu] [--serve-seconds 10] [--no-serve]
-                    [--legs plain,shard,e2e,batch,serve,cpu] [--copy-probe BYTES]
+    python bench.py [--gpus 1] [--steps 2000] [--warmup 200] [--no-cpu] [--serve-seconds 30] [--no-serve]
+                    [--legs plain,points,shard,e2e,batch,serve,cpu] [--copy-probe BYTES]
+
+Before the timed steps the multiply runs for at least SETTLE_S seconds (and
+--warmup steps), so the clocks have settled whatever --warmup is; the timed
+steps are exactly --steps.
 
 Prints one JSON line.
 """
@@ -35,10 +39,15 @@ SEED = 20260
 HBM_PEAK_GBS = 8000.0
 PCIE_PEAK_GBS = 64.0  # PCIe Gen5 x16, per direction (vendor figure)
 # 32-bit integer VALU: 1024 SIMDs x 16 lanes x 2.4 GHz lane-operations per second;
-# a Shoup butterfly (modmul, add, sub, all canonical) is 12 of them.
+# a Shoup butterfly (modmul, add, sub, all canonical) is 12 of them. This is the
+# nominal figure; the measured one (pm_probe_valu) prices the butterfly by the
+# measured issue rates of its 9 add-class instructions, 1 v_mul_hi_u32 and 2
+# v_mul_lo_u32.
 VALU_INT32_OPS = 1024 * 16 * 2.4e9
 BUTTERFLY_VALU = 12
 PEAK_MODMUL_PER_S = VALU_INT32_OPS / BUTTERFLY_VALU
+SETTLE_S = 0.3  # time-based warm-up before the timed steps (clocks settle)
+KERNEL_TIMING_LAUNCHES = 100  # launches per kernel in the HIP-event kernel times
 METRIC = "mults/s at N=65536/W=1747 (plain vs batched), %HBM/PCIe peak, serve p99 ms"
 KERNELS = ["rns", "col_fwd", "row_fused", "col_inv", "crt"]
 
@@ -105,10 +114,14 @@ def batched(pmlib, O, count: int, points, seed: int, check: int, iters: int, tun
     tuned = B.autotune(da, db, dc, 3) if tune else "default configuration"
     B.run(da, db, dc)
     B.sync()
-    exact = True
-    for i in range(min(check, count)):
+    from concurrent.futures import ThreadPoolExecutor
+
+    def exact_one(i):
         n, w, q = prods[i]
-        exact &= bool(np.array_equal(dc[i].download((n, O.limbs_of(w))), O.OraclePlan(n, w, q).multiply(*ops[i])))
+        return bool(np.array_equal(dc[i].download((n, O.limbs_of(w))), O.OraclePlan(n, w, q).multiply(*ops[i])))
+
+    with ThreadPoolExecutor(max_workers=8) as ex:  # the oracle's ctypes calls release the GIL
+        exact = all(ex.map(exact_one, range(min(check, count))))
     B.time(da, db, dc, 10)  # warm-up: the clocks settle after the oracle check
     ms = B.time(da, db, dc, iters)
     B.close()
@@ -139,6 +152,82 @@ def batched(pmlib, O, count: int, points, seed: int, check: int, iters: int, tun
             "plain_per_call_mults_per_s": round(count / (plain_total * 1e-3), 1),
             "speedup_vs_plain": round(plain_total / ms, 3),
             "io_bytes": io, "bit_exact_checked": min(check, count), "bit_exact": exact, "autotune": tuned}
+
+
+def point_line(pmlib, O, n: int, w: int, kinds, seed: int, e2e_calls: int = 20) -> dict:
+    """One plain point: for each operand distribution, the device-resident
+    result and the host-buffer (pm_run) result checked against the oracle,
+    then the device-resident rate (at least 0.2 s of back-to-back calls after
+    SETTLE_S of warm-up) and the median host-buffer call on the first."""
+    q = O.spec_q(n, w)
+    L = O.limbs_of(w)
+    P = O.OraclePlan(n, w, q)
+    out = {"N": n, "W": w, "L": L, "operands": list(kinds), "bit_exact": {}}
+    with pmlib.Multiplier(n, w, O.q_limbs(q, L)) as m:
+        out["K"] = m.K
+        dev = None
+        for i, kind in enumerate(kinds):
+            a, b = O.operands(kind, n, w, q, seed + i)
+            da, db = pmlib.DeviceBuffer.from_array(a), pmlib.DeviceBuffer.from_array(b)
+            dc = pmlib.DeviceBuffer(n * L * 4)
+            m.run_device(da, db, dc)
+            m.sync()
+            want = P.multiply(a, b)
+            out["bit_exact"][kind] = bool(np.array_equal(dc.download((n, L)), want)) and \
+                bool(np.array_equal(m.run(a, b), want))
+            if dev is None:
+                dev = (a, b, da, db, dc)
+        a, b, da, db, dc = dev
+
+        def calls(k):
+            for _ in range(k):
+                m.run_device(da, db, dc)
+            m.sync()
+        t = time.perf_counter()
+        while time.perf_counter() - t < SETTLE_S:
+            calls(20)
+        k, t = 0, time.perf_counter()
+        while k < 100 or time.perf_counter() - t < 0.2:
+            calls(20)
+            k += 20
+        dt = time.perf_counter() - t
+        out["device_mults_per_s"] = round(k / dt, 1)
+        c_host = np.empty((n, L), np.uint32)
+        m.run(a, b, c_host)
+        ts = []
+        for _ in range(e2e_calls):
+            t = time.perf_counter()
+            m.run(a, b, c_host)
+            ts.append(time.perf_counter() - t)
+        med = float(np.median(ts))
+        out["end_to_end_median_ms"] = round(1e3 * med, 4)
+        out["end_to_end_mults_per_s"] = round(1.0 / med, 1)
+        out["pcie_bytes"] = 12 * n * L
+    return out
+
+
+def cpu_point(O, n: int, w: int, cases: int, seed: int) -> dict | None:
+    """The reference's solve.cu on its CPU emulator, pinned to one core, at one
+    point: mults/s over `cases` uniform products (fherma_run times from its
+    results.json), one output checked against the oracle."""
+    exe = O.ref_binary()
+    if exe is None:
+        return None
+    q = O.spec_q(n, w)
+    with tempfile.TemporaryDirectory() as d:
+        ops = [O.operands("uniform", n, w, q, seed + i) for i in range(cases)]
+        O.write_point(d, n, w, q, ops)
+        core = sorted(os.sched_getaffinity(0))[0]
+        subprocess.run([str(exe), d], check=True, capture_output=True,
+                       preexec_fn=lambda: os.sched_setaffinity(0, {core}))
+        res = O.read_results(d)
+        secs = [c["seconds"] for c in res["cases"] if c["status"] == "ok"]
+        ok = bool(np.array_equal(O.read_output(d, 0, n, O.limbs_of(w)), O.OraclePlan(n, w, q).multiply(*ops[0])))
+    total = sum(secs)
+    return {"value": round(len(secs) / total, 2) if total else None, "unit": "mults/s", "cores": 1,
+            "kind": "reference", "sample": "%d uniform products at N=%d/W=%d, solve.cu on pm_emu.h, one core"
+                                          % (len(secs), n, w),
+            "output_matches_oracle": ok}
 
 
 SERVE_POINTS = [(8192, 438), (32768, 868), (65536, 1747)]
@@ -210,9 +299,9 @@ def main():
     ap.add_argument("--e2e-calls", type=int, default=20)
     ap.add_argument("--cpu-cases", type=int, default=3)
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--serve-seconds", type=float, default=10.0)
+    ap.add_argument("--serve-seconds", type=float, default=30.0)
     ap.add_argument("--no-serve", action="store_true")
-    ap.add_argument("--legs", default="plain,shard,e2e,batch,serve,cpu",
+    ap.add_argument("--legs", default="plain,points,shard,e2e,batch,serve,cpu",
                     help="comma list of parts to run (plain is always run: it gives value)")
     ap.add_argument("--no-autotune", action="store_true", help="batched legs on the default configuration")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="BYTES",
@@ -249,9 +338,15 @@ def main():
     c_dev = dc.download((N, L))
     correct = bool(np.array_equal(c_dev, O.OraclePlan(N, W, q).multiply(a, b)))
 
-    for _ in range(args.warmup):
-        m.run_device(da, db, dc)
-    m.sync()
+    # settle: at least SETTLE_S seconds and --warmup steps before the clock
+    t_settle = time.perf_counter()
+    done = 0
+    while done < args.warmup or time.perf_counter() - t_settle < SETTLE_S:
+        for _ in range(50):
+            m.run_device(da, db, dc)
+        m.sync()
+        done += 50
+    settle_ms = 1e3 * (time.perf_counter() - t_settle)
     t0 = time.perf_counter()
     for _ in range(args.steps):
         m.run_device(da, db, dc)
@@ -261,7 +356,7 @@ def main():
     value = args.steps / dt
 
     # per-kernel times from HIP events on the stream the kernels run on
-    ms = m.time_device(da, db, dc, max(10, min(args.steps, 50)))
+    ms = m.time_device(da, db, dc, KERNEL_TIMING_LAUNCHES)
     kms = dict(zip(KERNELS, ms[1:]))
     kb = kernel_bytes(K, L)
     dom = max(KERNELS, key=lambda k: kms[k])
@@ -272,12 +367,35 @@ def main():
         traffic = json.loads(tf.read_text()).get(dom)
     roofline = {"bound": "hbm", "kernel": dom, "achieved": round(ach, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                 "frac": round(ach / HBM_PEAK_GBS, 4), "traffic": traffic,
-                "algorithmic_bytes": kb[dom], "kernel_ms": round(kms[dom], 4)}
+                "traffic_source": "profiles/traffic.json: rocprofv3 --pmc FETCH_SIZE and WRITE_SIZE of this kernel, "
+                                  "scaled by the copy-kernel calibration (not measured in this run)",
+                "algorithmic_bytes": kb[dom], "kernel_ms": round(kms[dom], 4), "timed_launches": KERNEL_TIMING_LAUNCHES}
     mm = modmuls(K)
+    # the same kernel against the VALU: butterflies per second over the
+    # butterfly rate the measured issue rates allow
+    valu = pmlib.probe_valu(256)
+    t_bfly = 9 / valu["add_u32"] + 1 / valu["mul_hi_u32"] + 2 / valu["mul_lo_u32"]  # 1e-12 s per lane
+    peak_bfly = 1e12 / t_bfly
+    roofline_valu = {"bound": "valu", "kernel": dom, "achieved": round(mm[dom] / (kms[dom] * 1e-3) / 1e9, 1),
+                     "peak": round(peak_bfly / 1e9, 1), "unit": "G butterflies/s",
+                     "frac": round(mm[dom] / (kms[dom] * 1e-3) / peak_bfly, 4),
+                     "measured_issue_Tops": valu,
+                     "butterfly": "9 add-class + 1 v_mul_hi_u32 + 2 v_mul_lo_u32 lane instructions (Shoup CT)",
+                     "nominal_peak_G": round(PEAK_MODMUL_PER_S / 1e9, 1)}
     per_kernel = {k: {"ms": round(kms[k], 4), "GBps": round(kb[k] / (kms[k] * 1e-3) / 1e9, 1),
                       "Gmodmul_per_s": round(mm[k] / (kms[k] * 1e-3) / 1e9, 1),
-                      "modmul_frac_of_valu_peak": round(mm[k] / (kms[k] * 1e-3) / PEAK_MODMUL_PER_S, 3)}
+                      "modmul_frac_of_valu_peak": round(mm[k] / (kms[k] * 1e-3) / PEAK_MODMUL_PER_S, 3),
+                      "modmul_frac_of_measured_valu_peak": round(mm[k] / (kms[k] * 1e-3) / peak_bfly, 3)}
                   for k in KERNELS}
+
+    # the other plain points: C1 (the CPU-reference comparison point) and C2
+    # (four operand sets)
+    points = None
+    if "points" in legs:
+        points = {"C1": point_line(pmlib, O, 4096, 256, ["uniform"], SEED + 11),
+                  "C2": point_line(pmlib, O, 32768, 868, ["uniform", "max", "sparse", "small16"], SEED + 21)}
+        if "cpu" in legs:
+            points["C1"]["cpu_baseline"] = cpu_point(O, 4096, 256, 100, SEED + 31)
 
     # prime shards: what one GPU of an 8-GPU prime-sharded multiply runs
     shard = None
@@ -327,10 +445,10 @@ def main():
     batch_same = batch_mix = None
     if "batch" in legs:
         tune = not args.no_autotune
-        batch_same = batched(pmlib, O, 16, [(N, W)] * 16, SEED + 100, 2, 50, tune)
+        batch_same = batched(pmlib, O, 16, [(N, W)] * 16, SEED + 100, 16, 50, tune)
         rng = np.random.default_rng(SEED)
         mix = [(int(2 ** rng.integers(11, 17)), int(rng.integers(128, 2049))) for _ in range(64)]
-        batch_mix = batched(pmlib, O, 64, mix, SEED + 200, 4, 50, tune)
+        batch_mix = batched(pmlib, O, 64, mix, SEED + 200, 64, 50, tune)
         batch_mix["workload"] = "64 products, N log-uniform in 2^11..2^16, W uniform in [128, 2048], seed %d" % SEED
 
     serve = serving(pmlib, O, (200, 1000, 5000), args.serve_seconds, plain_serve) if "serve" in legs else None
@@ -338,7 +456,7 @@ def main():
     cpu = cpu_baseline(q, args.cpu_cases) if "cpu" in legs else None
     out = {
         "metric": METRIC, "value": round(value, 2), "unit": "mults/s", "n_gpus": 1, "steps": args.steps,
-        "warmup": args.warmup, "ms_per_step": round(ms_step, 4), "higher_is_better": True, "scaling": "weak",
+        "warmup": args.warmup, "settle_ms": round(settle_ms, 1), "ms_per_step": round(ms_step, 4), "higher_is_better": True, "scaling": "weak",
         "vs_baseline": None, "dtype": "u32", "data": "synthetic",
         "config": {"workload": "plain single product N=65536 W=1747 L=55 K=%d, mixed operands, "
                                "device-resident (pm_run_device)" % K, "N": N, "W": W, "L": L, "K": K,
@@ -348,8 +466,10 @@ def main():
         # M = K (3 (N/2) log2 N + 2N) modmuls per product, over the throughput, against the VALU peak
         "modmuls_per_product": sum(mm.values()),
         "pct_valu_modmul_peak": round(100 * sum(mm.values()) / (ms_step * 1e-3) / PEAK_MODMUL_PER_S, 2),
+        "pct_measured_valu_modmul_peak": round(100 * sum(mm.values()) / (ms_step * 1e-3) / peak_bfly, 2),
         "serial_multiply_ms": round(ms[0], 4),
-        "roofline": roofline, "kernels": per_kernel, "end_to_end": end_to_end, "prime_shard": shard,
+        "roofline": roofline, "roofline_valu": roofline_valu, "kernels": per_kernel, "end_to_end": end_to_end,
+        "points": points, "prime_shard": shard,
         "batched_same_point": batch_same, "batched_mixed": batch_mix, "serving": serve, "cpu_baseline": cpu,
     }
     print(json.dumps(out))

@@ -130,6 +130,14 @@ int pm_time_shard(pm_state* s, const uint32_t* d_a, const uint32_t* d_b, uint32_
  * No counterpart in the reference. */
 int pm_probe_copy(size_t bytes, int iters, double* gbps);
 
+/* Issue rates of four VALU instructions, in tera lane-operations per
+ * second: out[0] v_add_u32, out[1] v_mul_lo_u32, out[2] v_mul_hi_u32,
+ * out[3] v_mad_u64_u32 (8 waves per SIMD, 8 independent chains per lane,
+ * `iters` x 128 instructions per lane). Diagnostic: the measured peak the NTT
+ * kernels' modular multiplies are priced against. No counterpart in the
+ * reference. */
+int pm_probe_valu(int iters, double* out);
+
 /* ---- stage entry points, for stage-by-stage parity tests ----
  * All on device buffers, synchronous.
  *   rns:     R[K][N] = a mod p_j
@@ -225,6 +233,9 @@ typedef struct {
     double cpu_util;           /* process CPU seconds per wall second during the run */
     uint64_t ctx_invol;        /* involuntary context switches of the process */
     uint64_t driver_ctx_invol; /* ... of the driving thread */
+    /* results compared bit for bit with the plain path's product of the same
+     * operands (pm_init / pm_run), and how many differed */
+    uint64_t verified, mismatches;
 } pm_serve_report;
 int pm_server_create(uint32_t npoints, const uint32_t* N, const uint32_t* W, const uint32_t* q_concat,
                      const pm_server_config* cfg, pm_server** out);
@@ -241,7 +252,11 @@ int pm_server_wait(pm_server* s, uint64_t ticket, double timeout_s, double* late
 int pm_server_stats_get(pm_server* s, int reset, pm_server_stats* out);
 /* Open-loop Poisson load from the calling thread: arrivals at `rate` per
  * second for `seconds`, point i drawn with weight weights[i], operands
- * generated from `seed`; latency counts from each scheduled arrival. */
+ * uniform in [0, q) generated from `seed` (two pairs per point, alternating);
+ * latency counts from each scheduled arrival. Before the clock starts, the
+ * plain path (pm_init / pm_run) computes the product of every pair; each
+ * completed request's result is compared with it (report: verified,
+ * mismatches). */
 int pm_server_drive(pm_server* s, double rate, double seconds, const double* weights, uint64_t seed,
                     pm_serve_report* out);
 /* Closed-loop load: `depth` requests kept in flight for `seconds`, a new one

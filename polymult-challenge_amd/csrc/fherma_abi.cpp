@@ -2,10 +2,13 @@
 // the C ABI: same signatures and types, errors thrown as std::runtime_error.
 //
 // fherma_run must return a fresh N x L std::vector, and value-initialising
-// 14 MB costs milliseconds of page faults. A helper thread therefore builds
-// the next result vector between calls; the call only takes it. The helper
-// never sees operand data.
+// 14 MB costs milliseconds of page faults. So each call hands back a vector
+// built ahead of time: after a multiply, the call orders the next one from a
+// worker thread (a promise in the worker's queue) and the following call
+// collects it through the matching future. The worker never sees operand data.
 #include <condition_variable>
+#include <deque>
+#include <future>
 #include <memory>
 #include <mutex>
 #include <thread>
@@ -15,58 +18,73 @@
 
 namespace {
 
-class NextResult {
+using Result = std::vector<uint32_t>;
+
+class ResultSupply {
   public:
-    explicit NextResult(size_t words) : words_(words), th_([this] { loop(); }) { ask(); }
-    ~NextResult() {
+    explicit ResultSupply(size_t words) : words_(words), worker_([this] { serve(); }) { pending_ = order(); }
+    ~ResultSupply() {
         {
-            std::lock_guard<std::mutex> lk(m_);
-            stop_ = true;
+            std::lock_guard<std::mutex> g(mu_);
+            closing_ = true;
         }
-        cv_.notify_all();
-        th_.join();
+        wake_.notify_one();
+        worker_.join();
     }
-    std::vector<uint32_t> take() {
-        std::unique_lock<std::mutex> lk(m_);
-        cv_.wait(lk, [&] { return ready_; });
-        ready_ = false;
-        return std::move(buf_);
-    }
-    void ask() {
-        {
-            std::lock_guard<std::mutex> lk(m_);
-            wanted_ = true;
+    // The vector ordered last; the caller orders the next one with refill().
+    // If building it failed, the error reaches the caller and a new order
+    // keeps the next call going.
+    Result collect() {
+        try {
+            return pending_.get();
+        } catch (...) {
+            pending_ = order();
+            throw;
         }
-        cv_.notify_all();
     }
+    void refill() { pending_ = order(); }
 
   private:
-    void loop() {
-        std::unique_lock<std::mutex> lk(m_);
+    std::future<Result> order() {
+        std::promise<Result> pr;
+        std::future<Result> f = pr.get_future();
+        {
+            std::lock_guard<std::mutex> g(mu_);
+            orders_.push_back(std::move(pr));
+        }
+        wake_.notify_one();
+        return f;
+    }
+    void serve() {
         for (;;) {
-            cv_.wait(lk, [&] { return stop_ || wanted_; });
-            if (stop_) return;
-            wanted_ = false;
-            lk.unlock();
-            std::vector<uint32_t> v(words_);
-            lk.lock();
-            buf_ = std::move(v);
-            ready_ = true;
-            cv_.notify_all();
+            std::promise<Result> pr;
+            {
+                std::unique_lock<std::mutex> g(mu_);
+                wake_.wait(g, [&] { return closing_ || !orders_.empty(); });
+                if (orders_.empty()) return;  // closing, nothing left to fill
+                pr = std::move(orders_.front());
+                orders_.pop_front();
+            }
+            try {
+                pr.set_value(Result(words_));
+            } catch (...) {
+                pr.set_exception(std::current_exception());  // bad_alloc reaches the caller
+            }
         }
     }
-    size_t words_;
-    std::mutex m_;
-    std::condition_variable cv_;
-    bool wanted_ = false, ready_ = false, stop_ = false;
-    std::vector<uint32_t> buf_;
-    std::thread th_;
+    const size_t words_;
+    std::mutex mu_;
+    std::condition_variable wake_;
+    std::deque<std::promise<Result>> orders_;
+    bool closing_ = false;
+    std::future<Result> pending_;
+    std::thread worker_;
 };
 
 struct FhermaState {
     pm_state* s = nullptr;
     uint32_t N = 0, L = 0;
-    std::unique_ptr<NextResult> next;
+    std::unique_ptr<ResultSupply> next;
     ~FhermaState() {
         next.reset();  // stop the helper before the state goes
         pm_free(s);
@@ -83,7 +101,7 @@ void* fherma_init(const fherma::Point& pt) {
     F->s = s;
     F->N = pt.N;
     F->L = pt.L;
-    F->next = std::make_unique<NextResult>((size_t)pt.N * pt.L);
+    F->next = std::make_unique<ResultSupply>((size_t)pt.N * pt.L);
     return F.release();
 }
 
@@ -94,9 +112,9 @@ fherma::Outputs fherma_run(void* state, const fherma::Inputs& in) {
         throw std::runtime_error("operand size does not match N x L");
     fherma::Outputs out;
     out.c.shape = {static_cast<int64_t>(F->N), static_cast<int64_t>(F->L)};
-    out.c.data = F->next->take();
+    out.c.data = F->next->collect();
     const int rc = pm_run(F->s, in.a.data.data(), in.b.data.data(), out.c.data.data());
-    F->next->ask();
+    F->next->refill();
     if (rc != 0) throw std::runtime_error(pm_last_error());
     return out;
 }

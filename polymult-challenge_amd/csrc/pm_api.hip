@@ -179,21 +179,52 @@ void State::enqueue_lanes(const uint32_t* a, const uint32_t* b, uint32_t* c) {
     uint32_t* R0 = lane_R[L][0];
     uint32_t* R1 = lane_R[L][1];
     if (L == 0) acquire_R(st);  // lane 0 works in d_R
+    wait_for_inputs(st, L, a, b);
     hip_check(launch_rns(dev, cfg, a, b, R0, R1, 2, st), "rns");
     hip_check(launch_col(dev, cfg, R0, R1, 2, false, st), "col");
     hip_check(launch_row(dev, cfg, R1, R0, ROW_FUSED, st), "row");
     hip_check(launch_col(dev, cfg, R1, nullptr, 1, true, st), "col inv");
     if (prev_lane >= 0 && prev_lane != L) hip_check(hipStreamWaitEvent(st, ev_crt[prev_lane], 0), "wait");
     hip_check(launch_crt(dev, cfg, R1, c, dev.G, st), "crt");
+    note_output(lane_written[L], ev_crt[L], c);  // before ev_crt[L] moves on to this call
     hip_check(hipEventRecord(ev_crt[L], st), "record");
     if (L == 0) release_R(st);
     prev_lane = L;
+}
+
+namespace {
+bool overlaps(const State::Range& r, uintptr_t lo, uintptr_t hi) { return r.first < hi && lo < r.second; }
+}  // namespace
+
+void State::wait_for_inputs(hipStream_t st, int lane, const uint32_t* a, const uint32_t* b) {
+    const uintptr_t n = coeff_words * 4;
+    const uintptr_t a0 = (uintptr_t)a, b0 = (uintptr_t)b;
+    auto hit = [&](const std::vector<Range>& list) {
+        for (const Range& r : list)
+            if (overlaps(r, a0, a0 + n) || overlaps(r, b0, b0 + n)) return true;
+        return false;
+    };
+    for (int L = 0; L < kMaxLanes; ++L)
+        if (L != lane && ev_crt[L] && hit(lane_written[L])) hip_check(hipStreamWaitEvent(st, ev_crt[L], 0), "wait");
+    // lane 0 and the default-residue calls already wait on ev_R (acquire_R)
+    if (lane > 0 && R_used && hit(R_written)) hip_check(hipStreamWaitEvent(st, ev_R, 0), "wait");
+}
+
+void State::note_output(std::vector<Range>& list, hipEvent_t ev_before, const uint32_t* c) {
+    // ev_before still marks the end of the previous call that this list
+    // covers: once it has completed, none of the listed outputs is pending.
+    if (!list.empty() && hipEventQuery(ev_before) == hipSuccess) list.clear();
+    const Range r{(uintptr_t)c, (uintptr_t)c + coeff_words * 4};
+    if (std::find(list.begin(), list.end(), r) == list.end()) list.push_back(r);
 }
 
 void State::sync_all() {
     hip_check(hipStreamSynchronize(stream), "synchronize");
     for (int L = 1; L < kMaxLanes; ++L)
         if (lane_stream[L]) hip_check(hipStreamSynchronize(lane_stream[L]), "synchronize");
+    if (R_used) hip_check(hipEventSynchronize(ev_R), "synchronize");
+    for (auto& w : lane_written) w.clear();
+    R_written.clear();
 }
 
 std::vector<uint32_t> State::shard_bounds(int shards) const {
@@ -351,9 +382,13 @@ int pm_run_device(pm_state* s, const uint32_t* d_a, const uint32_t* d_b, uint32_
         if (!s || !d_a || !d_b || !d_c) throw std::runtime_error("null argument");
         if (stream) {
             hipStream_t st = static_cast<hipStream_t>(stream);
-            s->s->acquire_R(st);
-            hip_check(s->s->enqueue_multiply(d_a, d_b, d_c, st), "multiply");
-            s->s->release_R(st);
+            pm::State& S = *s->s;
+            S.acquire_R(st);
+            S.wait_for_inputs(st, -1, d_a, d_b);
+            hip_check(S.enqueue_multiply(d_a, d_b, d_c, st), "multiply");
+            if (S.R_used) S.note_output(S.R_written, S.ev_R, d_c);  // before ev_R moves on
+            else S.R_written.assign(1, {(uintptr_t)d_c, (uintptr_t)d_c + S.coeff_words * 4});
+            S.release_R(st);
         } else
             s->s->enqueue_lanes(d_a, d_b, d_c);
     });
@@ -533,6 +568,47 @@ int pm_probe_copy(size_t bytes, int iters, double* gbps) {
         (void)hipEventDestroy(e1);
         (void)hipFree(src);
         (void)hipFree(dst);
+    });
+}
+
+int pm_probe_valu(int iters, double* out) {
+    return guarded([&] {
+        if (!out || iters < 1) throw std::runtime_error("bad VALU probe arguments");
+        int dev = 0, cus = 0;
+        hip_check(hipGetDevice(&dev), "hipGetDevice");
+        hip_check(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev), "attribute");
+        const int grid = cus * 8;  // 8 blocks of 4 waves per CU: 8 waves per SIMD
+        uint32_t* d = nullptr;
+        hipEvent_t e0 = nullptr, e1 = nullptr;
+        hipStream_t st = nullptr;
+        auto release = [&] {
+            if (e0) (void)hipEventDestroy(e0);
+            if (e1) (void)hipEventDestroy(e1);
+            if (st) (void)hipStreamDestroy(st);
+            if (d) (void)hipFree(d);
+        };
+        try {
+            hip_check(hipMalloc(&d, (size_t)grid * 256 * 4), "hipMalloc");
+            hip_check(hipStreamCreateWithFlags(&st, hipStreamNonBlocking), "stream");
+            hip_check(hipEventCreate(&e0), "event");
+            hip_check(hipEventCreate(&e1), "event");
+            for (int op = 0; op < 4; ++op) {
+                hip_check(pm::launch_valu_probe(op, d, grid, iters, st), "probe");  // warm
+                constexpr int reps = 5;
+                hip_check(hipEventRecord(e0, st), "record");
+                for (int r = 0; r < reps; ++r) hip_check(pm::launch_valu_probe(op, d, grid, iters, st), "probe");
+                hip_check(hipEventRecord(e1, st), "record");
+                hip_check(hipEventSynchronize(e1), "sync");
+                float ms = 0;
+                hip_check(hipEventElapsedTime(&ms, e0, e1), "elapsed");
+                const double lane_ops = (double)grid * 256 * iters * pm::kValuProbeOpsPerIter * reps;
+                out[op] = lane_ops / (ms * 1e-3) / 1e12;
+            }
+        } catch (...) {
+            release();
+            throw;
+        }
+        release();
     });
 }
 

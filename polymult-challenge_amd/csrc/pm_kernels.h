@@ -74,6 +74,12 @@ hipError_t launch_crt_partial(const DevPlan& D, const LaunchCfg& C, const uint32
 hipError_t launch_crt_combine(const DevPlan& D, const LaunchCfg& C, const long long* P, int shards, size_t shard_stride,
                               uint32_t* out, hipStream_t s);
 
+// VALU issue-rate probe: `grid` blocks of 256 threads, each thread 8 chains
+// of 128 instructions per iteration of kind op (0 v_add_u32, 1 v_mul_lo_u32,
+// 2 v_mul_hi_u32, 3 v_mad_u64_u32); out[grid * 256] keeps the results.
+constexpr int kValuProbeOpsPerIter = 128;
+hipError_t launch_valu_probe(int op, uint32_t* out, int grid, int iters, hipStream_t s);
+
 // dst = src, `bytes` a multiple of 16, on `grid` blocks of 256 threads.
 hipError_t launch_copy_probe(const void* src, void* dst, size_t bytes, int grid, hipStream_t s);
 

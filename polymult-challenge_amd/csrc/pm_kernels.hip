@@ -841,6 +841,40 @@ __global__ void __launch_bounds__(256) k_copy_probe(const uint4* __restrict__ sr
         dst[i] = src[i];
 }
 
+// VALU issue-rate probe: every thread runs 8 independent chains of one
+// instruction (0 v_add_u32, 1 v_mul_lo_u32, 2 v_mul_hi_u32, 3
+// v_mad_u64_u32), 16 x 8 per loop pass, so enough waves per SIMD hide each
+// chain's latency and the time measures issue rate. Written as asm so the
+// compiler can neither fold the chains nor change the instruction; they touch
+// registers only. out gets one word per thread (keeps the results live).
+template <int OP>
+__global__ void __launch_bounds__(256) k_valu_probe(u32* __restrict__ out, u32 seed, int iters) {
+    const u32 tid = blockIdx.x * blockDim.x + threadIdx.x;
+    u32 x[8];
+    u64 y[8];
+#pragma unroll
+    for (int i = 0; i < 8; ++i) {
+        x[i] = seed + tid * 8u + (u32)i;
+        y[i] = x[i];
+    }
+    const u32 k = seed | 1u;
+    for (int it = 0; it < iters; ++it) {
+#pragma unroll
+        for (int u = 0; u < 16; ++u)
+#pragma unroll
+            for (int i = 0; i < 8; ++i) {
+                if constexpr (OP == 0) asm volatile("v_add_u32 %0, %1, %0" : "+v"(x[i]) : "v"(k));
+                else if constexpr (OP == 1) asm volatile("v_mul_lo_u32 %0, %0, %1" : "+v"(x[i]) : "v"(k));
+                else if constexpr (OP == 2) asm volatile("v_mul_hi_u32 %0, %0, %1" : "+v"(x[i]) : "v"(k));
+                else asm volatile("v_mad_u64_u32 %0, vcc, %1, %2, %0" : "+v"(y[i]) : "v"(x[i]), "v"(k) : "vcc");
+            }
+    }
+    u32 r = 0;
+#pragma unroll
+    for (int i = 0; i < 8; ++i) r ^= x[i] ^ (u32)y[i] ^ (u32)(y[i] >> 32);
+    out[tid] = r;
+}
+
 __global__ void k_batch_reset(u32* ctl, u32 n) {
     for (u32 i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) ctl[i] = 0;
 }
@@ -1034,6 +1068,17 @@ hipError_t launch_copy_probe(const void* src, void* dst, size_t bytes, int grid,
     if (bytes % 16) return hipErrorInvalidValue;
     hipLaunchKernelGGL(k_copy_probe, dim3(grid), dim3(256), 0, s, static_cast<const uint4*>(src),
                        static_cast<uint4*>(dst), bytes / 16);
+    return hipGetLastError();
+}
+
+hipError_t launch_valu_probe(int op, u32* out, int grid, int iters, hipStream_t s) {
+    switch (op) {
+        case 0: hipLaunchKernelGGL((k_valu_probe<0>), dim3(grid), dim3(256), 0, s, out, 12345u, iters); break;
+        case 1: hipLaunchKernelGGL((k_valu_probe<1>), dim3(grid), dim3(256), 0, s, out, 12345u, iters); break;
+        case 2: hipLaunchKernelGGL((k_valu_probe<2>), dim3(grid), dim3(256), 0, s, out, 12345u, iters); break;
+        case 3: hipLaunchKernelGGL((k_valu_probe<3>), dim3(grid), dim3(256), 0, s, out, 12345u, iters); break;
+        default: return hipErrorInvalidValue;
+    }
     return hipGetLastError();
 }
 

@@ -631,23 +631,134 @@ void Server::complete_loop() {
 // ---------------------------------------------------------------------------
 namespace {
 
-// Random operand below 2^(W-1) < q: every limb uniform, the top bits cleared.
-std::vector<uint32_t> random_operand(std::mt19937_64& g, uint32_t N, uint32_t W) {
-    const uint32_t L = (W + 31) / 32;
-    std::vector<uint32_t> v((size_t)N * L);
-    for (uint32_t n = 0; n < N; ++n)
-        for (uint32_t i = 0; i < L; ++i) {
-            const int bits = std::max(0, std::min(32, (int)W - 1 - 32 * (int)i));
-            const uint32_t mask = bits == 32 ? 0xFFFFFFFFu : ((1u << bits) - 1u);
-            v[(size_t)n * L + i] = (uint32_t)g() & mask;
+// Operand uniform in [0, q): W-bit draws per coefficient, redrawn while >= q
+// (q > 2^(W-1), so fewer than half are redrawn; with q = 2^W - delta and a
+// small delta, almost none).
+std::vector<uint32_t> uniform_operand(std::mt19937_64& g, const HostPlan& P) {
+    const uint32_t L = P.L, top = P.W - 32 * (L - 1);  // bits of the top limb, 1..32
+    const uint32_t mask = top == 32 ? 0xFFFFFFFFu : ((1u << top) - 1u);
+    std::vector<uint32_t> v((size_t)P.N * L);
+    for (uint32_t n = 0; n < P.N; ++n) {
+        uint32_t* x = v.data() + (size_t)n * L;
+        for (bool below = false; !below;) {
+            for (uint32_t i = 0; i < L; ++i) x[i] = (uint32_t)g();
+            x[L - 1] &= mask;
+            int i = (int)L - 1;
+            while (i >= 0 && x[i] == P.q[i]) --i;
+            below = i >= 0 && x[i] < P.q[i];
         }
+    }
     return v;
 }
+
+// The operands a drive sends and the products it checks them against. The
+// products come from the plain path (pm_init / pm_run: the point's own prime
+// set and the single-product kernels), not from the batch phase kernels that
+// serve the requests.
+struct DriveSet {
+    std::vector<std::vector<std::vector<uint32_t>>> ops;   // [point][4]: a0, b0, a1, b1
+    std::vector<std::vector<std::vector<uint32_t>>> want;  // [point][2]: a0 b0, a1 b1
+};
+
+void plain_products(const HostPlan& P, DriveSet& D, size_t i) {
+    pm_state* st = nullptr;
+    if (pm_init(P.N, P.W, P.L, P.q.data(), P.L, &st) != 0)
+        throw std::runtime_error(std::string("expected products: ") + pm_last_error());
+    std::unique_ptr<pm_state, void (*)(pm_state*)> hold(st, pm_free);
+    for (int k = 0; k < 2; ++k) {
+        std::vector<uint32_t> c(D.ops[i][2 * k].size());
+        if (pm_run(st, D.ops[i][2 * k].data(), D.ops[i][2 * k + 1].data(), c.data()) != 0)
+            throw std::runtime_error(std::string("expected products: ") + pm_last_error());
+        D.want[i].push_back(std::move(c));
+    }
+}
+
+using Buf = std::unique_ptr<std::vector<uint32_t>>;
+
+// Result buffers per point, shared by the driving thread (takes) and the
+// checker (gives back).
+struct BufPool {
+    std::mutex m;
+    std::vector<std::vector<Buf>> free;
+    Buf take(size_t point, size_t words) {
+        {
+            std::lock_guard<std::mutex> lk(m);
+            if (!free[point].empty()) {
+                Buf b = std::move(free[point].back());
+                free[point].pop_back();
+                return b;
+            }
+        }
+        return std::make_unique<std::vector<uint32_t>>(words);
+    }
+    void give(size_t point, Buf b) {
+        std::lock_guard<std::mutex> lk(m);
+        free[point].push_back(std::move(b));
+    }
+};
+
+// Compares every finished request's result with its expected product on a
+// thread of its own, so the driving thread keeps to its arrival schedule,
+// then returns the buffer to the pool.
+class Checker {
+  public:
+    Checker(const DriveSet& D, BufPool& pool) : D_(D), pool_(pool), th_([this] { loop(); }) {}
+    ~Checker() { finish(); }
+    void push(uint32_t point, int pair, Buf b) {
+        {
+            std::lock_guard<std::mutex> lk(m_);
+            q_.push_back({point, pair, std::move(b)});
+        }
+        cv_.notify_one();
+    }
+    // Checks what is queued, then stops the thread.
+    void finish() {
+        {
+            std::lock_guard<std::mutex> lk(m_);
+            if (done_) return;
+            done_ = true;
+        }
+        cv_.notify_one();
+        th_.join();
+    }
+    uint64_t checked = 0, mismatches = 0;
+
+  private:
+    struct Item {
+        uint32_t point;
+        int pair;
+        Buf buf;
+    };
+    void loop() {
+        for (;;) {
+            Item it;
+            {
+                std::unique_lock<std::mutex> lk(m_);
+                cv_.wait(lk, [&] { return done_ || !q_.empty(); });
+                if (q_.empty()) return;
+                it = std::move(q_.front());
+                q_.pop_front();
+            }
+            const std::vector<uint32_t>& w = D_.want[it.point][it.pair];
+            if (it.buf->size() != w.size() || std::memcmp(it.buf->data(), w.data(), w.size() * 4) != 0) ++mismatches;
+            ++checked;
+            pool_.give(it.point, std::move(it.buf));
+        }
+    }
+    const DriveSet& D_;
+    BufPool& pool_;
+    std::mutex m_;
+    std::condition_variable cv_;
+    std::deque<Item> q_;
+    bool done_ = false;
+    std::thread th_;
+};
 
 // Open loop (depth 0): Poisson arrivals at `rate`, latency from each
 // scheduled arrival. Closed loop (depth > 0): `depth` requests kept in
 // flight, a new one submitted as soon as one completes; the achieved rate is
-// then the server's capacity on the mix.
+// then the server's capacity on the mix. Every completed result is compared
+// with the plain path's product of the same operands.
 void drive(Server& S, double rate, uint32_t depth, double seconds, const double* weights, uint64_t seed,
            pm_serve_report* out) {
     if ((depth == 0 && !(rate > 0)) || !(seconds > 0)) throw std::runtime_error("rate and seconds must be positive");
@@ -660,40 +771,88 @@ void drive(Server& S, double rate, uint32_t depth, double seconds, const double*
     }
     if (!(wsum > 0)) throw std::runtime_error("mix weights sum to zero");
     std::mt19937_64 g(seed);
-    // two operand pairs per point, alternating over the requests of that point
-    std::vector<std::vector<std::vector<uint32_t>>> ops(np);
+    // two operand pairs per point, alternating over the requests; kept on the
+    // heap, so that requests a failed drive could not wait for can still read
+    // them (see Drain)
+    auto D = std::make_shared<DriveSet>();
+    D->ops.resize(np);
+    D->want.resize(np);
     for (size_t i = 0; i < np; ++i)
-        if (w[i] > 0)
-            for (int k = 0; k < 4; ++k) ops[i].push_back(random_operand(g, S.cat.plans[i].N, S.cat.plans[i].W));
+        if (w[i] > 0) {
+            for (int k = 0; k < 4; ++k) D->ops[i].push_back(uniform_operand(g, S.cat.plans[i]));
+            plain_products(S.cat.plans[i], *D, i);
+        }
     // Result buffers, allocated and touched before the clock starts (a first
     // touch inside the run would charge page faults to the latencies); the
     // pool grows only if more requests of a point are outstanding at once.
-    std::vector<std::vector<std::unique_ptr<std::vector<uint32_t>>>> pool(np);
+    auto pool = std::make_shared<BufPool>();
+    pool->free.resize(np);
     for (size_t i = 0; i < np; ++i)
         if (w[i] > 0)
             for (int k = (int)std::ceil((depth ? depth : 64) * w[i] / wsum) + 2; k > 0; --k)
-                pool[i].push_back(std::make_unique<std::vector<uint32_t>>(S.cat.io_words(i), 1u));
+                pool->free[i].push_back(std::make_unique<std::vector<uint32_t>>(S.cat.io_words(i), 1u));
     std::discrete_distribution<int> pick(w.begin(), w.end());
     std::exponential_distribution<double> gap(depth ? 1.0 : rate);
+    Checker check(*D, *pool);
 
     struct Out {
         uint64_t id;
         uint32_t point;
-        std::unique_ptr<std::vector<uint32_t>> buf;
+        int pair;
+        Buf buf;
     };
     std::deque<Out> outstanding;
+    // On an error (a request still undone after 60 s, a refused submission)
+    // the server may still read the operands and write the result buffers of
+    // the requests in flight: wait for each of them before anything is freed,
+    // and if one still does not finish, keep its memory for the life of the
+    // process rather than free it under the server.
+    struct Drain {
+        Server& S;
+        std::deque<Out>& outstanding;
+        std::shared_ptr<DriveSet> D;
+        ~Drain() {
+            bool leak = false;
+            for (Out& o : outstanding) {
+                int rc;
+                try {
+                    rc = S.wait(o.id, 60.0, nullptr, nullptr);
+                } catch (...) {
+                    rc = -1;
+                }
+                if (rc == 1) {
+                    leak = true;
+                    (void)o.buf.release();
+                }
+            }
+            if (leak) (void)new std::shared_ptr<DriveSet>(D);
+        }
+    } drain{S, outstanding, D};
     std::vector<double> lat, qlat, slat;
     std::vector<std::vector<double>> plat(np);
     uint64_t failed = 0, nreq = 0;
     double max_late = 0;
+    auto finished = [&](Out& o, int rc, double ms, double qms) {
+        if (rc == 0) {
+            lat.push_back(ms);
+            qlat.push_back(qms);
+            slat.push_back(ms - qms);
+            plat[o.point].push_back(ms);
+            check.push(o.point, o.pair, std::move(o.buf));
+        } else {
+            ++failed;
+            pool->give(o.point, std::move(o.buf));
+        }
+    };
     // Requests finish in batch order, so reaping stops at the first one not
     // done (or waits for each, at the end).
     auto reap = [&](bool block) {
-        for (auto it = outstanding.begin(); it != outstanding.end();) {
+        while (!outstanding.empty()) {
+            Out& o = outstanding.front();
             double ms = 0, qms = 0;
             int rc;
             try {
-                rc = S.wait(it->id, block ? 60.0 : 0.0, &ms, &qms);
+                rc = S.wait(o.id, block ? 60.0 : 0.0, &ms, &qms);
             } catch (const std::exception&) {
                 rc = -1;
             }
@@ -701,16 +860,8 @@ void drive(Server& S, double rate, uint32_t depth, double seconds, const double*
                 if (block) throw std::runtime_error("request did not complete within 60 s");
                 break;
             }
-            if (rc == 0) {
-                lat.push_back(ms);
-                qlat.push_back(qms);
-                slat.push_back(ms - qms);
-                plat[it->point].push_back(ms);
-            } else {
-                ++failed;
-            }
-            pool[it->point].push_back(std::move(it->buf));
-            it = outstanding.erase(it);
+            finished(o, rc, ms, qms);
+            outstanding.pop_front();
         }
     };
 
@@ -721,13 +872,10 @@ void drive(Server& S, double rate, uint32_t depth, double seconds, const double*
     getrusage(RUSAGE_THREAD, &rt0);
     auto submit_one = [&](double t_arrive) {
         const uint32_t p = (uint32_t)pick(g);
-        if (pool[p].empty())
-            pool[p].push_back(std::make_unique<std::vector<uint32_t>>(S.cat.io_words(p)));
-        Out o{0, p, std::move(pool[p].back())};
-        pool[p].pop_back();
-        const auto& op = ops[p];
-        const size_t k = (size_t)(nreq % 2) * 2;
-        o.id = S.submit(p, op[k].data(), op[k + 1].data(), o.buf->data(), t_arrive);
+        const int pair = (int)(nreq % 2);
+        Out o{0, p, pair, pool->take(p, S.cat.io_words(p))};
+        const auto& op = D->ops[p];
+        o.id = S.submit(p, op[2 * pair].data(), op[2 * pair + 1].data(), o.buf->data(), t_arrive);
         outstanding.push_back(std::move(o));
         ++nreq;
     };
@@ -746,15 +894,7 @@ void drive(Server& S, double rate, uint32_t depth, double seconds, const double*
                 rc = -1;
             }
             if (rc == 1) throw std::runtime_error("request did not complete within 60 s");
-            if (rc == 0) {
-                lat.push_back(ms);
-                qlat.push_back(qms);
-                slat.push_back(ms - qms);
-                plat[f.point].push_back(ms);
-            } else {
-                ++failed;
-            }
-            pool[f.point].push_back(std::move(f.buf));
+            finished(f, rc, ms, qms);
             outstanding.pop_front();
             reap(false);
         }
@@ -775,6 +915,7 @@ void drive(Server& S, double rate, uint32_t depth, double seconds, const double*
     }
     reap(true);
     const double t_end = now_s();
+    check.finish();
     rusage ru1{}, rt1{};
     getrusage(RUSAGE_SELF, &ru1);
     getrusage(RUSAGE_THREAD, &rt1);
@@ -823,6 +964,8 @@ void drive(Server& S, double rate, uint32_t depth, double seconds, const double*
     out->scatter_ms = st.scatter_ms;
     out->sched_busy = st.sched_busy;
     out->completer_busy = st.completer_busy;
+    out->verified = check.checked;
+    out->mismatches = check.mismatches;
 }
 
 }  // namespace

@@ -5,6 +5,7 @@
 
 #include <cstddef>
 #include <cstdint>
+#include <utility>
 #include <vector>
 
 #include "pm_kernels.h"
@@ -61,6 +62,18 @@ struct State {
     int next_lane = 0;
     void enqueue_lanes(const uint32_t* a, const uint32_t* b, uint32_t* c);
     void sync_all();
+
+    // Read after write across lanes and streams: the outputs written by calls
+    // that may still be in flight, per event that orders them (a lane's
+    // ev_crt, or ev_R for the calls that use the default residues). A call
+    // whose a or b overlaps one of them first waits for that event, so a
+    // result can feed the next call (x = a b, then y = x b) without pm_sync.
+    using Range = std::pair<uintptr_t, uintptr_t>;  // [first byte, end)
+    std::vector<Range> lane_written[kMaxLanes];
+    std::vector<Range> R_written;
+    void wait_for_inputs(hipStream_t st, int lane, const uint32_t* a, const uint32_t* b);
+    // Adds c's range to `list`; call before `ev_before` is recorded again.
+    void note_output(std::vector<Range>& list, hipEvent_t ev_before, const uint32_t* c);
 
     // Prime shards (the multi-GPU split, run here one shard after another):
     // boundaries j_0 = 0 < j_1 < ... < j_G = K on whole MFMA k-steps (8

@@ -25,6 +25,7 @@ C_SYMBOLS = (
     "pm_stream_destroy", "pm_sync", "pm_time_device",
     "pm_shard_bounds", "pm_run_device_sharded", "pm_time_shard",
     "pm_stage_rns", "pm_stage_ntt_fwd", "pm_stage_ntt_inv", "pm_stage_crt", "pm_probe_transfers", "pm_probe_copy",
+    "pm_probe_valu",
     "pm_batch_create", "pm_batch_free", "pm_batch_last_error", "pm_batch_product_info", "pm_batch_run",
     "pm_batch_sync", "pm_batch_time", "pm_batch_autotune", "pm_batch_configure",
     "pm_server_create", "pm_server_free", "pm_server_last_error", "pm_server_submit", "pm_server_wait",
@@ -73,7 +74,8 @@ class ServeReport(ctypes.Structure):
                                                "sched_busy", "completer_busy", "queue_p50_ms", "queue_p99_ms",
                                                "service_p50_ms", "service_p99_ms")] + \
                [("point_p50_ms", ctypes.c_double * 8), ("point_p99_ms", ctypes.c_double * 8),
-                ("cpu_util", ctypes.c_double), ("ctx_invol", ctypes.c_uint64), ("driver_ctx_invol", ctypes.c_uint64)]
+                ("cpu_util", ctypes.c_double), ("ctx_invol", ctypes.c_uint64), ("driver_ctx_invol", ctypes.c_uint64),
+                ("verified", ctypes.c_uint64), ("mismatches", ctypes.c_uint64)]
 
 
 def _as_dict(st: ctypes.Structure) -> dict:
@@ -123,6 +125,7 @@ def load() -> ctypes.CDLL:
         "pm_probe_transfers": ([ctypes.c_size_t, ctypes.c_int, ctypes.c_int, ctypes.POINTER(ctypes.c_double)],
                                ctypes.c_int),
         "pm_probe_copy": ([ctypes.c_size_t, ctypes.c_int, ctypes.POINTER(ctypes.c_double)], ctypes.c_int),
+        "pm_probe_valu": ([ctypes.c_int, ctypes.POINTER(ctypes.c_double)], ctypes.c_int),
         "pm_shard_bounds": ([_vp, ctypes.c_int, _u32p], ctypes.c_int),
         "pm_run_device_sharded": ([_vp, _vp, _vp, _vp, ctypes.c_int, _vp], ctypes.c_int),
         "pm_time_shard": ([_vp, _vp, _vp, _vp, ctypes.c_int, ctypes.c_int, ctypes.POINTER(ctypes.c_float)],
@@ -196,6 +199,14 @@ def probe_copy(nbytes: int, iters: int = 10) -> float:
     g = ctypes.c_double()
     _check(load().pm_probe_copy(nbytes, iters, ctypes.byref(g)))
     return g.value
+
+
+def probe_valu(iters: int = 256) -> dict:
+    """Issue rates (tera lane-operations/s) of v_add_u32, v_mul_lo_u32,
+    v_mul_hi_u32 and v_mad_u64_u32 (see pm_probe_valu)."""
+    out = (ctypes.c_double * 4)()
+    _check(load().pm_probe_valu(iters, out))
+    return dict(zip(("add_u32", "mul_lo_u32", "mul_hi_u32", "mad_u64_u32"), [round(v, 3) for v in out]))
 
 
 class DeviceBuffer:

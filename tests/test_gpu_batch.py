@@ -70,6 +70,35 @@ def test_batch_log_uniform_mix(O, pm):
     _check(O, prods, ops, dc)
 
 
+def test_batch_64_product_workload(O, pm):
+    """The batched workload as bench.py times it: 64 products, N log-uniform in
+    2^11..2^16, W uniform in [128, 2048], operands uniform in [0, q), one graph
+    launch after autotuning. Every one of the 64 products is compared with the
+    oracle (computed on host threads: the oracle's ctypes calls release the GIL)."""
+    from concurrent.futures import ThreadPoolExecutor
+    rng = np.random.default_rng(20260)
+    points = [(int(2 ** rng.integers(11, 17)), int(rng.integers(128, 2049))) for _ in range(64)]
+    print("batch of 64:", points, flush=True)
+    B, prods, ops, da, db, dc = _batch(O, pm, points, 20260 + 200)
+    best = B.autotune(da, db, dc, 2)
+    print("autotune", best, flush=True)
+    for d in dc:
+        d.upload(np.zeros(d.nbytes // 4, np.uint32))
+    B.run(da, db, dc)
+    B.sync()
+
+    def want(i):
+        N, W, q = prods[i]
+        return O.OraclePlan(N, W, q).multiply(*ops[i])
+
+    with ThreadPoolExecutor(max_workers=8) as ex:
+        wants = list(ex.map(want, range(64)))
+    bad = [i for i in range(64) if not np.array_equal(dc[i].download((prods[i][0], O.limbs_of(prods[i][1]))),
+                                                       wants[i])]
+    print("64 products checked, %d differ" % len(bad), flush=True)
+    assert not bad, "products differing from the oracle: %s" % bad
+
+
 def test_batch_closed_form_full_size(O, pm):
     """Eight N=65536/W=1747 products of all-(q-1) operands: c_k = (2k + 2 - N) mod q each."""
     N, W = 65536, 1747

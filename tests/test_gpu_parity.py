@@ -226,6 +226,46 @@ def test_device_calls_in_flight(O, pm):
         assert np.array_equal(m.run(*ops[1]), dc[1].download((N, L)))
 
 
+def test_device_calls_chained(O, pm):
+    """Device calls that read earlier calls' outputs with no host wait: on the
+    default stream (the lanes) x1 = a b, x2 = x1 b, x3 = x2 x1, x4 = x3 a, three
+    times over; then a caller-stream result read by a default-stream call and
+    the other way round. Each read waits for the write it depends on."""
+    N, W = 32768, 868
+    q = O.spec_q(N, W)
+    L = O.limbs_of(W)
+    a, b = O.operands("uniform", N, W, q, 1300)
+    P = O.OraclePlan(N, W, q)
+    x1 = P.multiply(a, b)
+    x2 = P.multiply(x1, b)
+    x3 = P.multiply(x2, x1)
+    x4 = P.multiply(x3, a)
+    s = pm.Stream()
+    with pm.Multiplier(N, W, O.q_limbs(q, L)) as m:
+        da, db = pm.DeviceBuffer.from_array(a), pm.DeviceBuffer.from_array(b)
+        dx = [pm.DeviceBuffer(N * L * 4) for _ in range(6)]
+        for rep in range(3):
+            for d in dx:
+                d.upload(np.zeros((N, L), np.uint32))
+            print("chain round", rep, flush=True)
+            m.run_device(da, db, dx[0])
+            m.run_device(dx[0], db, dx[1])
+            m.run_device(dx[1], dx[0], dx[2])
+            m.run_device(dx[2], da, dx[3])
+            m.run_device(dx[3], dx[3], dx[4], s.handle)  # caller stream after the lanes
+            m.run_device(dx[4], da, dx[5])  # lanes after the caller stream
+            m.sync()
+            s.sync()
+            assert np.array_equal(dx[0].download((N, L)), x1)
+            assert np.array_equal(dx[1].download((N, L)), x2)
+            assert np.array_equal(dx[2].download((N, L)), x3)
+            assert np.array_equal(dx[3].download((N, L)), x4)
+            x5 = P.multiply(x4, x4)
+            assert np.array_equal(dx[4].download((N, L)), x5)
+            assert np.array_equal(dx[5].download((N, L)), P.multiply(x5, a))
+    s.close()
+
+
 def test_time_device_leaves_product(O, pm):
     """pm_time_device repeats each kernel on the chain's residues, then runs
     whole multiplies: its times are positive and d_c ends holding a * b."""

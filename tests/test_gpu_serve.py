@@ -82,14 +82,16 @@ def test_serve_admission_under_tiny_budget(O, pm):
 
 
 def test_serve_poisson_drive_short(O, pm):
-    """One second of open-loop Poisson load on the serving mix: every request
-    completes, none fails, percentiles are ordered."""
+    """Two seconds of open-loop Poisson load on the serving mix: every request
+    completes, none fails, every result equals the plain path's product of
+    its operands (checked by the driver), percentiles are ordered."""
     cat = _catalog(O, MIX_POINTS[:3])
     with pm.Server(cat) as S:
-        rep = S.drive(500.0, 1.0, [0.70, 0.25, 0.05], seed=7)
+        rep = S.drive(500.0, 2.0, [0.70, 0.25, 0.05], seed=7)
         print("drive", rep, flush=True)
-        assert rep["requests"] > 300
+        assert rep["requests"] > 700
         assert rep["completed"] == rep["requests"] and rep["failed"] == 0
+        assert rep["verified"] == rep["completed"] and rep["mismatches"] == 0
         assert 0 < rep["p50_ms"] <= rep["p99_ms"] <= rep["max_ms"]
         # the server stays usable and exact after the load
         N, W, q = cat[0]
@@ -105,7 +107,21 @@ def test_serve_saturate_closed_loop(O, pm):
         rep = S.saturate(16, 0.5, [0.70, 0.25, 0.05], seed=3)
         print("saturate", {k: rep[k] for k in ("requests", "completed", "achieved_rate", "mean_batch")}, flush=True)
         assert rep["completed"] == rep["requests"] > 16 and rep["failed"] == 0
+        assert rep["verified"] == rep["completed"] and rep["mismatches"] == 0
         assert rep["mean_batch"] > 1.0
+
+
+def test_serve_drive_expected_products_match_oracle(O, pm):
+    """The driver's reference for each result is the plain path (pm_run) at the
+    serving points; pin that path against the oracle at those points, on
+    operands uniform in [0, q) as the driver draws them."""
+    for N, W in MIX_POINTS[:3]:
+        q = O.spec_q(N, W)
+        L = O.limbs_of(W)
+        a, b = O.operands("uniform", N, W, q, 4242 + N)
+        with pm.Multiplier(N, W, O.q_limbs(q, L)) as m:
+            got = m.run(a, b)
+        assert np.array_equal(got, O.OraclePlan(N, W, q).multiply(a, b)), (N, W)
 
 
 def test_serve_concurrent_submitters(O, pm):
