@@ -60,6 +60,22 @@ int pm_run(pm_state* s, const uint32_t* a, const uint32_t* b, uint32_t* c);
  * the device-resident form of the same call. */
 int pm_run_device(pm_state* s, const uint32_t* d_a, const uint32_t* d_b, uint32_t* d_c, void* stream);
 
+/* The same multiply as pm_run_device, with no allocation on the call: its two
+ * residue arrays live in a fixed array in the library's device code (96 MiB
+ * per GPU, held from the library's load to the end of the process), so the
+ * call makes no hipMalloc, hipFree or other allocation, and no event wait or
+ * record. No counterpart in the reference: it is for callers that cannot
+ * allocate (code under HIP graph capture, tight serving loops, harnesses
+ * that forbid allocation). Limit: 2 K N <= 25,165,824 residue words (K N <=
+ * 12,582,912; N = 65536 with up to 192 primes, W up to about 2,950 bits);
+ * larger points fail with a message. Runs in order on `stream` (NULL: the
+ * state's own stream); it is not ordered against the state's calls on other
+ * streams or lanes (pm_sync, or the caller's stream, orders them). Every call
+ * on one GPU, from any state, uses that GPU's one array: calls that overlap on
+ * a GPU race. Nothing in the array needs clearing between calls (each call
+ * writes every residue it reads). */
+int pm_run_device_scratch(pm_state* s, const uint32_t* d_a, const uint32_t* d_b, uint32_t* d_c, void* stream);
+
 /* Releases everything. Replaces fherma_free (fherma.h:48; solve.cu:1531-1535). */
 void pm_free(pm_state* s);
 

@@ -8,6 +8,7 @@
 #include <cstring>
 #include <functional>
 #include <memory>
+#include <mutex>
 #include <stdexcept>
 #include <string>
 #include <vector>
@@ -48,6 +49,49 @@ struct TableUploader {
         return off;
     }
 };
+
+}  // namespace
+
+// ---------------------------------------------------------------------------
+// Residues of the allocation-free device call (pm_run_device_scratch): one
+// array per GPU, in the code object, so the call needs no device allocation.
+// Each GPU holds it from the library's load until the process ends. In host
+// code `scratch` names a host-side stand-in, never the device memory: the
+// device address comes from hipGetSymbolAddress, looked up once per device
+// (at pm_init, with that device current) and kept in g_scratch.
+// ---------------------------------------------------------------------------
+constexpr size_t kScratchWords = (size_t)24 << 20;  // 96 MiB: both residue arrays, 2 K N words
+static __device__ __align__(256) uint32_t scratch[kScratchWords];
+
+namespace {
+
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+struct ScratchSlot {
+    bool looked_up = false;
+    uint32_t* ptr = nullptr;
+    hipError_t err = hipSuccess;
+};
+ScratchSlot g_scratch[kMaxDevices];
+
+// The device address of `device`'s scratch array (the device must be
+// current), or nullptr with *err set.
+uint32_t* scratch_address(int device, hipError_t* err) {
+    if (device < 0 || device >= kMaxDevices) {
+        *err = hipErrorInvalidDevice;
+        return nullptr;
+    }
+    std::lock_guard<std::mutex> lk(g_scratch_mu);
+    ScratchSlot& S = g_scratch[device];
+    if (!S.looked_up) {
+        void* p = nullptr;
+        S.err = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        S.ptr = S.err == hipSuccess ? static_cast<uint32_t*>(p) : nullptr;
+        S.looked_up = true;
+    }
+    *err = S.err;
+    return S.ptr;
+}
 
 }  // namespace
 
@@ -106,6 +150,7 @@ void State::init(uint32_t N, uint32_t W, uint32_t L, const uint32_t* q, size_t q
     plan_point(host, N, W, L, q, q_limbs);
     build_tables(host);
     hip_check(hipGetDevice(&device), "hipGetDevice");
+    d_scratch = scratch_address(device, &scratch_err);  // checked by pm_run_device_scratch
     int max_lds = 0;
     hip_check(hipDeviceGetAttribute(&max_lds, hipDeviceAttributeMaxSharedMemoryPerBlock, device), "device attribute");
     hip_check(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking), "stream");
@@ -391,6 +436,24 @@ int pm_run_device(pm_state* s, const uint32_t* d_a, const uint32_t* d_b, uint32_
             S.release_R(st);
         } else
             s->s->enqueue_lanes(d_a, d_b, d_c);
+    });
+}
+
+int pm_run_device_scratch(pm_state* s, const uint32_t* d_a, const uint32_t* d_b, uint32_t* d_c, void* stream) {
+    return guarded([&] {
+        if (!s || !d_a || !d_b || !d_c) throw std::runtime_error("null argument");
+        pm::State& S = *s->s;
+        const size_t kn = (size_t)S.host.K * S.host.N;
+        if (2 * kn > kScratchWords)
+            throw std::runtime_error("point too large for the allocation-free call: its residues need " +
+                                     std::to_string(2 * kn * 4) + " bytes, the scratch array holds " +
+                                     std::to_string(kScratchWords * 4) + " (2 K N <= " +
+                                     std::to_string(kScratchWords) + " words)");
+        if (!S.d_scratch)
+            throw std::runtime_error(std::string("scratch array unavailable on this device: ") +
+                                     hipGetErrorString(S.scratch_err));
+        hipStream_t st = stream ? static_cast<hipStream_t>(stream) : S.stream;
+        hip_check(S.enqueue_multiply_with(d_a, d_b, d_c, st, S.d_scratch, S.d_scratch + kn), "multiply");
     });
 }
 

@@ -30,6 +30,10 @@ struct State {
     size_t device_bytes = 0;
     Pipeline* pipe = nullptr;
     long long* d_part = nullptr;  // prime-shard CRT partials, grown on demand
+    // This device's scratch array (the allocation-free call's residues), or
+    // nullptr and the lookup's error.
+    uint32_t* d_scratch = nullptr;
+    hipError_t scratch_err = hipSuccess;
     size_t part_bytes = 0;
     // d_R and d_part serve the caller-stream calls, lane 0, the host path, the
     // sharded path and the timing helpers, on whatever stream each runs. Every

@@ -20,7 +20,7 @@ LIB_PATH = HERE / "libpolymult.so"
 
 # Every C symbol include/polymult.h declares.
 C_SYMBOLS = (
-    "pm_init", "pm_run", "pm_run_device", "pm_free", "pm_last_error", "pm_query", "pm_point_info",
+    "pm_init", "pm_run", "pm_run_device", "pm_run_device_scratch", "pm_free", "pm_last_error", "pm_query", "pm_point_info",
     "pm_dev_alloc", "pm_dev_free", "pm_copy_to_device", "pm_copy_to_host", "pm_stream_create", "pm_stream_sync",
     "pm_stream_destroy", "pm_sync", "pm_time_device",
     "pm_shard_bounds", "pm_run_device_sharded", "pm_time_shard",
@@ -104,6 +104,7 @@ def load() -> ctypes.CDLL:
                      ctypes.POINTER(_vp)], ctypes.c_int),
         "pm_run": ([_vp, _u32p, _u32p, _u32p], ctypes.c_int),
         "pm_run_device": ([_vp, _vp, _vp, _vp, _vp], ctypes.c_int),
+        "pm_run_device_scratch": ([_vp, _vp, _vp, _vp, _vp], ctypes.c_int),
         "pm_free": ([_vp], None),
         "pm_last_error": ([], ctypes.c_char_p),
         "pm_query": ([_vp, ctypes.POINTER(PmInfo)], ctypes.c_int),
@@ -315,6 +316,10 @@ class Multiplier:
         consecutive calls overlap on internal lanes (results still land
         in call order); call sync() before reading dc or changing da, db."""
         _check(load().pm_run_device(self._h, da.ptr, db.ptr, dc.ptr, stream))
+
+    def run_device_scratch(self, da: DeviceBuffer, db: DeviceBuffer, dc: DeviceBuffer, stream=None):
+        """The allocation-free device call (pm_run_device_scratch)."""
+        _check(load().pm_run_device_scratch(self._h, da.ptr, db.ptr, dc.ptr, stream))
 
     def sync(self):
         """Waits for every device call of this multiplier."""
