@@ -121,6 +121,8 @@ void State::upload_plan() {
     D.logN = H.logN;
     D.w1 = H.w1;
     D.w2 = H.w2;
+    D.npass = H.npass;
+    for (int i = 0; i < 4; ++i) D.pw[i] = H.pw[i];
     D.K = H.K;
     D.L = (int)H.L;
     D.W = (int)H.W;
@@ -163,8 +165,9 @@ void State::init(uint32_t N, uint32_t W, uint32_t L, const uint32_t* q, size_t q
     for (int L = 1; L < lanes; ++L)
         hip_check(hipStreamCreateWithFlags(&lane_stream[L], hipStreamNonBlocking), "stream");
     upload_plan();
-    cfg = choose_launch(dev, std::min<size_t>((size_t)max_lds, 64 * 1024));
-    if (cfg.crt_lds > (size_t)max_lds || cfg.rns_lds > (size_t)max_lds)
+    cfg = choose_launch(dev, std::min<size_t>((size_t)max_lds, 64 * 1024), (size_t)max_lds);
+    if (cfg.crt_lds > (size_t)max_lds || cfg.rns_lds > (size_t)max_lds ||
+        crt_combine_lds_bytes(cfg.crt_tn, (int)L, cfg.combine_direct) > (size_t)max_lds)
         throw std::runtime_error("point too large for this device's LDS");
     coeff_words = (size_t)N * L;
     const size_t rbytes = (size_t)host.K * N * 4;
@@ -174,6 +177,10 @@ void State::init(uint32_t N, uint32_t W, uint32_t L, const uint32_t* q, size_t q
     }
     hip_check(hipMalloc(&d_out, coeff_words * 4), "hipMalloc result");
     device_bytes = table_bytes + 2 * rbytes + 3 * coeff_words * 4;
+    if (cfg.crt_split) {
+        hip_check(hipMalloc(&d_slots, slot_bytes()), "hipMalloc CRT slots");
+        device_bytes += slot_bytes();
+    }
     pipeline_init();
     // Warm-up on zero operands: loads every code object, touches every buffer.
     hip_check(hipMemsetAsync(d_in[0], 0, coeff_words * 4, stream), "memset");
@@ -192,17 +199,17 @@ void State::release_R(hipStream_t st) {
 }
 
 hipError_t State::enqueue_multiply(const uint32_t* a, const uint32_t* b, uint32_t* c, hipStream_t st) {
-    return enqueue_multiply_with(a, b, c, st, d_R[0], d_R[1]);
+    return enqueue_multiply_with(a, b, c, st, d_R[0], d_R[1], d_slots);
 }
 
 hipError_t State::enqueue_multiply_with(const uint32_t* a, const uint32_t* b, uint32_t* c, hipStream_t st,
-                                        uint32_t* R0, uint32_t* R1) {
+                                        uint32_t* R0, uint32_t* R1, long long* slots) {
     hipError_t e;
     if ((e = launch_rns(dev, cfg, a, b, R0, R1, 2, st)) != hipSuccess) return e;
     if ((e = launch_col(dev, cfg, R0, R1, 2, false, st)) != hipSuccess) return e;
     if ((e = launch_row(dev, cfg, R1, R0, ROW_FUSED, st)) != hipSuccess) return e;
     if ((e = launch_col(dev, cfg, R1, nullptr, 1, true, st)) != hipSuccess) return e;
-    return launch_crt(dev, cfg, R1, c, dev.G, st);
+    return launch_crt(dev, cfg, R1, c, dev.G, st, 0, 0, slots);
 }
 
 void State::enqueue_lanes(const uint32_t* a, const uint32_t* b, uint32_t* c) {
@@ -214,9 +221,14 @@ void State::enqueue_lanes(const uint32_t* a, const uint32_t* b, uint32_t* c) {
             lane_stream[0] = stream;
             lane_R[0][0] = d_R[0];
             lane_R[0][1] = d_R[1];
+            lane_slots[0] = d_slots;
         } else {
             for (int i = 0; i < 2; ++i) hip_check(hipMalloc(&lane_R[L][i], rbytes), "hipMalloc residues");
             device_bytes += 2 * rbytes;
+            if (cfg.crt_split) {
+                hip_check(hipMalloc(&lane_slots[L], slot_bytes()), "hipMalloc CRT slots");
+                device_bytes += slot_bytes();
+            }
         }
         hip_check(hipEventCreateWithFlags(&ev_crt[L], hipEventDisableTiming), "event");
     }
@@ -230,7 +242,7 @@ void State::enqueue_lanes(const uint32_t* a, const uint32_t* b, uint32_t* c) {
     hip_check(launch_row(dev, cfg, R1, R0, ROW_FUSED, st), "row");
     hip_check(launch_col(dev, cfg, R1, nullptr, 1, true, st), "col inv");
     if (prev_lane >= 0 && prev_lane != L) hip_check(hipStreamWaitEvent(st, ev_crt[prev_lane], 0), "wait");
-    hip_check(launch_crt(dev, cfg, R1, c, dev.G, st), "crt");
+    hip_check(launch_crt(dev, cfg, R1, c, dev.G, st, 0, 0, lane_slots[L]), "crt");
     note_output(lane_written[L], ev_crt[L], c);  // before ev_crt[L] moves on to this call
     hip_check(hipEventRecord(ev_crt[L], st), "record");
     if (L == 0) release_R(st);
@@ -334,7 +346,7 @@ void State::enqueue_sharded(const uint32_t* a, const uint32_t* b, uint32_t* c, i
     hip_check(hipDeviceGetAttribute(&max_lds, hipDeviceAttributeMaxSharedMemoryPerBlock, device), "attribute");
     for (int g = 0; g < shards; ++g) {
         const DevPlan V = shard_view(bd[g], bd[g + 1] - bd[g]);
-        const LaunchCfg C = choose_launch(V, std::min<size_t>((size_t)max_lds, 64 * 1024));
+        const LaunchCfg C = choose_launch(V, std::min<size_t>((size_t)max_lds, 64 * 1024), (size_t)max_lds);
         hip_check(enqueue_shard(V, C, bd[g], a, b, d_part + (size_t)g * stride, st), "shard");
     }
     hip_check(launch_crt_combine(dev, cfg, d_part, shards, stride, c, st), "combine");
@@ -351,9 +363,11 @@ State::~State() {
     for (int L = 1; L < kMaxLanes; ++L) {
         for (auto& r : lane_R[L])
             if (r) (void)hipFree(r);
+        if (lane_slots[L]) (void)hipFree(lane_slots[L]);
         if (lane_stream[L]) (void)hipStreamDestroy(lane_stream[L]);
     }
     if (d_part) (void)hipFree(d_part);
+    if (d_slots) (void)hipFree(d_slots);
     pipeline_free();
     for (int i = 0; i < 2; ++i) {
         if (d_R[i]) (void)hipFree(d_R[i]);
@@ -444,16 +458,19 @@ int pm_run_device_scratch(pm_state* s, const uint32_t* d_a, const uint32_t* d_b,
         if (!s || !d_a || !d_b || !d_c) throw std::runtime_error("null argument");
         pm::State& S = *s->s;
         const size_t kn = (size_t)S.host.K * S.host.N;
-        if (2 * kn > kScratchWords)
+        // residues of a and b, then (points with the split CRT) its int64 slots
+        const size_t slot_words = S.cfg.crt_split ? S.slot_bytes() / 4 : 0;
+        if (2 * kn + slot_words > kScratchWords)
             throw std::runtime_error("point too large for the allocation-free call: its residues need " +
-                                     std::to_string(2 * kn * 4) + " bytes, the scratch array holds " +
+                                     std::to_string((2 * kn + slot_words) * 4) + " bytes, the scratch array holds " +
                                      std::to_string(kScratchWords * 4) + " (2 K N <= " +
                                      std::to_string(kScratchWords) + " words)");
         if (!S.d_scratch)
             throw std::runtime_error(std::string("scratch array unavailable on this device: ") +
                                      hipGetErrorString(S.scratch_err));
         hipStream_t st = stream ? static_cast<hipStream_t>(stream) : S.stream;
-        hip_check(S.enqueue_multiply_with(d_a, d_b, d_c, st, S.d_scratch, S.d_scratch + kn), "multiply");
+        long long* slots = slot_words ? reinterpret_cast<long long*>(S.d_scratch + 2 * kn) : nullptr;
+        hip_check(S.enqueue_multiply_with(d_a, d_b, d_c, st, S.d_scratch, S.d_scratch + kn, slots), "multiply");
     });
 }
 
@@ -519,7 +536,7 @@ int pm_time_device(pm_state* h, const uint32_t* d_a, const uint32_t* d_b, uint32
         ms[2] = time_runs([&] { hip_check(pm::launch_col(D, C, S.d_R[0], S.d_R[1], 2, false, st), "col"); });
         ms[3] = time_runs([&] { hip_check(pm::launch_row(D, C, S.d_R[1], S.d_R[0], pm::ROW_FUSED, st), "row"); });
         ms[4] = time_runs([&] { hip_check(pm::launch_col(D, C, S.d_R[1], nullptr, 1, true, st), "col inv"); });
-        ms[5] = time_runs([&] { hip_check(pm::launch_crt(D, C, S.d_R[1], d_c, D.G, st), "crt"); });
+        ms[5] = time_runs([&] { hip_check(pm::launch_crt(D, C, S.d_R[1], d_c, D.G, st, 0, 0, S.d_slots), "crt"); });
         // Whole multiplies last, so d_c ends holding a * b.
         ms[0] = time_runs([&] { hip_check(S.enqueue_multiply(d_a, d_b, d_c, st), "multiply"); });
         S.release_R(st);
@@ -556,7 +573,7 @@ int pm_time_shard(pm_state* h, const uint32_t* d_a, const uint32_t* d_b, uint32_
         hip_check(hipDeviceGetAttribute(&max_lds, hipDeviceAttributeMaxSharedMemoryPerBlock, S.device), "attribute");
         // shard 0 is a largest one
         const pm::DevPlan V = S.shard_view(bd[0], bd[1] - bd[0]);
-        const pm::LaunchCfg C = pm::choose_launch(V, std::min<size_t>((size_t)max_lds, 64 * 1024));
+        const pm::LaunchCfg C = pm::choose_launch(V, std::min<size_t>((size_t)max_lds, 64 * 1024), (size_t)max_lds);
         const size_t stride = (size_t)(S.dev.L + 3) * S.dev.N;
         uint32_t* R0 = S.d_R[0];
         uint32_t* R1 = S.d_R[1];
@@ -704,7 +721,7 @@ int pm_stage_ntt_inv(pm_state* h, uint32_t* d_R) {
 int pm_stage_crt(pm_state* h, const uint32_t* d_X, uint32_t* d_c) {
     return guarded([&] {
         pm::State& S = *h->s;
-        hip_check(pm::launch_crt(S.dev, S.cfg, d_X, d_c, S.dev.Gplain, S.stream), "crt");
+        hip_check(pm::launch_crt(S.dev, S.cfg, d_X, d_c, S.dev.Gplain, S.stream, 0, 0, S.d_slots), "crt");
         hip_check(hipStreamSynchronize(S.stream), "sync");
     });
 }

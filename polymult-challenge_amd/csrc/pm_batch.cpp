@@ -107,6 +107,9 @@ void Catalog::build(uint32_t count, const uint32_t* N, const uint32_t* W, const 
         D.logN = H.logN;
         D.w1 = H.w1;
         D.w2 = H.w2;
+        D.npass = 2;
+        D.pw[0] = D.w1;
+        D.pw[1] = D.w2;
         D.K = H.K;
         D.L = (int)H.L;
         D.W = (int)H.W;
@@ -157,6 +160,9 @@ void Runner::plan(const BatchConfig& c) {
         const int logN = D.logN;
         D.w1 = (logN + (c.ceil_split ? 1 : 0)) / 2;
         D.w2 = logN - D.w1;
+        D.npass = 2;
+        D.pw[0] = D.w1;
+        D.pw[1] = D.w2;
         const uint32_t nrows = 1u << D.w1, ncols = 1u << D.w2;
         const int Tc = line_threads(D.w1), Tr = line_threads(D.w2);
         B.col_tc = (int)std::min<uint32_t>({(uint32_t)c.col_tc, ncols, (uint32_t)(kBatchThreads[PH_COLF] / Tc)});
@@ -187,7 +193,7 @@ void Runner::plan(const BatchConfig& c) {
         } while (g1 < P && bytes + 8 * (size_t)prods[g1].D.K * prods[g1].D.N <= cap);
         if (g1 == g0 + 1 && c.single_kernels) {
             single.push_back((int)g0);
-            single_cfg.push_back(choose_launch(prods[g0].D, std::min<size_t>((size_t)max_lds, 64 * 1024)));
+            single_cfg.push_back(choose_launch(prods[g0].D, std::min<size_t>((size_t)max_lds, 64 * 1024), (size_t)max_lds));
             const LaunchCfg& L = single_cfg.back();
             if (L.crt_lds > (size_t)max_lds || L.rns_lds > (size_t)max_lds)
                 throw std::runtime_error("batch product too large for the device's LDS");

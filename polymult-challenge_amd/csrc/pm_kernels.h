@@ -12,7 +12,12 @@ typedef int v4i __attribute__((ext_vector_type(4)));
 // Everything a kernel needs, passed by value.
 struct DevPlan {
     uint32_t N;
+    // NTT passes: pw[0 .. npass-1] stages each (sum logN, each at most 8);
+    // the last is the row pass. w1 = pw[0], w2 = pw[npass-1] (two passes up to
+    // N = 2^16: the 4-step's columns and rows).
     int logN, w1, w2, K, L, W;
+    int npass;
+    int pw[4];
     uint64_t delta;
     const uint32_t* p;     // [K]
     const uint32_t* pinv;  // [K]
@@ -42,21 +47,29 @@ struct LaunchCfg {
     int crt_tn = 64;    // coefficients per CRT block
     int crt_ys = 0;     // LDS bytes per coefficient of y bytes
     size_t crt_lds = 0;
+    // The CRT block of y bytes and slots does not fit the device's LDS: the
+    // CRT runs as its linear part (slots to a global buffer of (L + 3) N
+    // int64) and the finish, which reads them from global memory when they do
+    // not fit the LDS either (combine_direct).
+    bool crt_split = false;
+    bool combine_direct = false;
     int col_tc = 32;    // columns per column-pass block
     int row_rpb = 16;   // rows per row-pass block
 };
 
 enum RowMode { ROW_FUSED = 0, ROW_FWD = 1, ROW_INV = 2 };
 
-// Chooses block shapes for a plan under the device's LDS limit.
-LaunchCfg choose_launch(const DevPlan& D, size_t max_lds);
+// Chooses block shapes for a plan: tiles sized to `max_lds`, the CRT split
+// when its block exceeds `device_lds` (the device's LDS per block).
+LaunchCfg choose_launch(const DevPlan& D, size_t max_lds, size_t device_lds);
 
 // R_i[j][n] = a_i[n] mod p_j for nops (1 or 2) operands, for the coefficients
 // n in [first, first + count) (count 0: to the end). a_i and R_i are the
 // whole arrays; `first` must be a multiple of 128.
 hipError_t launch_rns(const DevPlan& D, const LaunchCfg& C, const uint32_t* a0, const uint32_t* a1,
                       uint32_t* R0, uint32_t* R1, int nops, hipStream_t s, uint32_t first = 0, uint32_t count = 0);
-// Column pass (stages 0 .. w1-1, or their inverse) on nops residue arrays.
+// The strided passes (every pass but the row pass: stages 0 .. logN - w2 - 1,
+// or their inverse in reverse order) on nops residue arrays.
 hipError_t launch_col(const DevPlan& D, const LaunchCfg& C, uint32_t* R0, uint32_t* R1, int nops, bool inverse,
                       hipStream_t s);
 // Row pass. ROW_FUSED: RB <- inv(fwd(RB) * fwd(RA)) per row (RA untouched,
@@ -64,8 +77,9 @@ hipError_t launch_col(const DevPlan& D, const LaunchCfg& C, uint32_t* R0, uint32
 hipError_t launch_row(const DevPlan& D, const LaunchCfg& C, uint32_t* RB, const uint32_t* RA, RowMode mode,
                       hipStream_t s);
 // out[n][L] = CRT of R[j][n] (scaled by G) mod q, for n in [first, first + count).
+// With C.crt_split, `part` is the (L + 3) N int64 slot buffer.
 hipError_t launch_crt(const DevPlan& D, const LaunchCfg& C, const uint32_t* R, uint32_t* out, const uint2* G,
-                      hipStream_t s, uint32_t first = 0, uint32_t count = 0);
+                      hipStream_t s, uint32_t first = 0, uint32_t count = 0, long long* part = nullptr);
 
 // Prime shards: D a shard's view (its primes, whole k-steps). P[slot][n]
 // int64, L + 3 slots; the combine sums `shards` such arrays (shard_stride
@@ -124,6 +138,12 @@ inline size_t rns_lds_bytes(int tn, int L, int rns_rt) { return (size_t)tn * 4 *
 // {G lo, G hi, p, 0} for each of the 8 crt_ks primes of the k-steps.
 inline size_t crt_lds_bytes(int tn, int ys, int L, int crt_ks) {
     return (size_t)tn * ys + (size_t)tn * (L + 3) * 8 + (size_t)128 * crt_ks;
+}
+// The CRT's linear part alone (y bytes and constants) and the finish alone
+// (slots unless direct, then the L + 2 limbs of each coefficient).
+inline size_t crt_partial_lds_bytes(int tn, int ys, int crt_ks) { return (size_t)tn * ys + (size_t)128 * crt_ks; }
+inline size_t crt_combine_lds_bytes(int tn, int L, bool direct) {
+    return (size_t)tn * ((direct ? 0 : (size_t)(L + 3) * 8) + (size_t)(L + 2) * 4);
 }
 constexpr int kBatchCrtTile = 32;
 

@@ -157,22 +157,30 @@ __device__ __forceinline__ void inv_pass1(u32 (&x)[Line<NB>::E], const uint2* __
 }
 
 // ---------------------------------------------------------------------------
-// Column tile: columns [col0, col0 + tc) of prime j, tc T threads (threads
-// from tc T on only take part in the barriers). LDS tile [2^NB][tc] for the
-// transpose between the two register passes, then the table's first 2^NB
-// twiddle pairs: pass 2 reads its per-thread twiddles there (staged before
-// pass 1, behind the transpose barrier), not from global memory stage by
-// stage. Threads sweep columns fastest, so every global access of a wave
-// reads or writes whole row segments.
+// Strided tile: one pass of NB NTT stages, s0 .. s0 + NB - 1, over the lines
+// of block `hi` (hi < 2^s0) of prime j: position m of the line for column c
+// is hi 2^(NB + rest) + m 2^rest + c, m < 2^NB. The tile is columns [col0,
+// col0 + tc) (c < 2^rest), tc T threads (threads from tc T on only take part
+// in the barriers). With s0 = 0 (hi = 0, rest = w2) it is the 4-step's column
+// pass; N above 2^16 chains several such passes before the row pass.
+// LDS tile [2^NB][tc] for the transpose between the two register passes,
+// then the 2^NB - 2^R1 twiddle pairs pass 2 reads: stage ls of the line
+// needs entries 2^(s0+ls) + hi 2^ls + i, i < 2^ls (the inverse: the mirrored
+// line 2^s0 - 1 - hi), staged at 2^ls + i, which is where pass 2 looks with
+// s0 = 0 and line 0 (for s0 = 0 the staged copy is the table's first 2^NB
+// pairs). They are staged before pass 1, behind the transpose barrier, so
+// pass 2 does not wait on global memory stage by stage. Threads sweep
+// columns fastest, so every global access of a wave reads or writes whole
+// row segments.
 // ---------------------------------------------------------------------------
 // TC > 0 fixes the tile width at compile time (the plain path's 32 columns):
 // the LDS offsets of the unrolled accesses become immediates.
 template <int NB, bool INV, int TC = 0>
 __device__ __forceinline__ void col_tile(const DevPlan& D, u32* __restrict__ R, int j, int col0, int tc_in, u32* lds,
-                                         int tid) {
+                                         int tid, int s0, u32 hi, int rest) {
     const int tc = TC > 0 ? TC : tc_in;
     using Ln = Line<NB>;
-    constexpr int E = Ln::E, T = Ln::T, R2 = Ln::R2;
+    constexpr int E = Ln::E, T = Ln::T, R1 = Ln::R1, R2 = Ln::R2;
     const bool on = tid < tc * T;
     const int c = tid % tc, t = tid / tc;
     const u32 p = D.p[j];
@@ -180,19 +188,26 @@ __device__ __forceinline__ void col_tile(const DevPlan& D, u32* __restrict__ R, 
     // The tile's base is the same for the whole block; each access adds a
     // 32-bit byte offset to it (the prime's vector is below 2^32 bytes), one
     // 64-bit add per access instead of a sign extension, a 64-bit shift and
-    // an add. Row r of the tile sits r 2^w2 words on; the offsets are split
+    // an add. Line position m sits m 2^rest words on; the offsets are split
     // into the thread's part and a uniform step per k.
-    unsigned char* base = reinterpret_cast<unsigned char*>(R + (size_t)j * D.N + col0);
-    const u32 w2 = D.w2;
-    const u32 strided0 = (((u32)t << w2) + (u32)c) << 2;      // rows t + k T
-    const u32 blocked0 = (((u32)(t * E) << w2) + (u32)c) << 2;  // rows t E + k
-    auto strided = [&](int k) -> u32& { return *reinterpret_cast<u32*>(base + strided0 + ((u32)(k * T) << (w2 + 2))); };
-    auto blocked = [&](int k) -> u32& { return *reinterpret_cast<u32*>(base + blocked0 + ((u32)k << (w2 + 2))); };
+    unsigned char* base =
+        reinterpret_cast<unsigned char*>(R + (size_t)j * D.N + ((size_t)hi << (NB + rest)) + col0);
+    const u32 sh = (u32)rest;
+    const u32 strided0 = (((u32)t << sh) + (u32)c) << 2;      // positions t + k T
+    const u32 blocked0 = (((u32)(t * E) << sh) + (u32)c) << 2;  // positions t E + k
+    auto strided = [&](int k) -> u32& { return *reinterpret_cast<u32*>(base + strided0 + ((u32)(k * T) << (sh + 2))); };
+    auto blocked = [&](int k) -> u32& { return *reinterpret_cast<u32*>(base + blocked0 + ((u32)k << (sh + 2))); };
     u32 x[1][E];
     uint2* ltw = reinterpret_cast<uint2*>(lds + (1 << NB) * tc);
     auto stage_twiddles = [&] {
+        // entry 2^ls + i <- table (2^s0 + line) 2^ls + i
+        const u32 line = INV ? (1u << s0) - 1u - hi : hi;
+        const u32 lift = (1u << s0) + line - 1u;  // 0 for s0 = 0
         if (on)
-            for (int i = tid; i < (1 << NB); i += tc * T) ltw[i] = tab[i];
+            for (int i = (1 << R1) + tid; i < (1 << NB); i += tc * T) {
+                const int ls = 31 - __clz(i);
+                ltw[i] = tab[(u32)i + (lift << ls)];
+            }
     };
     if (!INV) {
         if (on) {
@@ -200,7 +215,7 @@ __device__ __forceinline__ void col_tile(const DevPlan& D, u32* __restrict__ R, 
             for (int k = 0; k < E; ++k) x[0][k] = strided(k);
         }
         if constexpr (R2 > 0) stage_twiddles();  // read after the transpose barrier
-        if (on) fwd_pass1<NB, 1>(x, tab, 0, 0, p);
+        if (on) fwd_pass1<NB, 1>(x, tab, s0, hi, p);
         if constexpr (R2 > 0) {
             if (on) {
 #pragma unroll
@@ -241,17 +256,21 @@ __device__ __forceinline__ void col_tile(const DevPlan& D, u32* __restrict__ R, 
             for (int k = 0; k < E; ++k) x[0][k] = strided(k);
         }
         if (on) {
-            inv_pass1<NB>(x[0], tab, 0, 0, p);
+            inv_pass1<NB>(x[0], tab, s0, hi, p);
 #pragma unroll
             for (int k = 0; k < E; ++k) strided(k) = x[0][k];
         }
     }
 }
 
+// Block x = hi 2^tiles_log + tile: 2^tiles_log column tiles of tc per line block.
 template <int NB, bool INV, int TC>
-__global__ void __launch_bounds__(1024) k_col(DevPlan D, u32* __restrict__ R0, u32* __restrict__ R1, int tc) {
+__global__ void __launch_bounds__(1024) k_col(DevPlan D, u32* __restrict__ R0, u32* __restrict__ R1, int tc, int s0,
+                                              int rest, int tiles_log) {
     extern __shared__ u32 lds[];
-    col_tile<NB, INV, TC>(D, blockIdx.z ? R1 : R0, (int)blockIdx.y, (int)blockIdx.x * tc, tc, lds, (int)threadIdx.x);
+    const u32 hi = blockIdx.x >> tiles_log, tile = blockIdx.x & ((1u << tiles_log) - 1u);
+    col_tile<NB, INV, TC>(D, blockIdx.z ? R1 : R0, (int)blockIdx.y, (int)(tile * (u32)tc), tc, lds, (int)threadIdx.x,
+                          s0, hi, rest);
 }
 
 // ---------------------------------------------------------------------------
@@ -277,7 +296,8 @@ __device__ __forceinline__ void row_tile(const DevPlan& D, u32* __restrict__ RB,
     constexpr int RS = (1 << NB) + (R2 > 0 ? 16 : 0);
     const bool on = tid < rpb * T;
     const int r = on ? tid / T : 0, t = tid % T;
-    const u32 nrows = 1u << D.w1;
+    const int s0 = D.logN - NB;  // stages before the row pass
+    const u32 nrows = 1u << s0;
     u32 row = row0 + (u32)r;
     if ((rpb & 1) == 0 && (u32)rpb < nrows) {
         const u32 h = (u32)rpb / 2, base = (row0 / (u32)rpb) * h;
@@ -287,7 +307,6 @@ __device__ __forceinline__ void row_tile(const DevPlan& D, u32* __restrict__ RB,
     const uint2* tab = D.tw + (size_t)j * D.tw_stride;
     const size_t off = (size_t)j * D.N + ((size_t)row << NB);
     u32* rb = RB + off;
-    const int s0 = D.w1;
     u32* lb = lds + r * RS;
     // padded LDS index of position t + k T (t < T, T a power of two up to 16:
     // (t + k T) / 16 = k T / 16) and of position 16 t + k (k < 16)
@@ -447,11 +466,15 @@ __device__ __forceinline__ long long slot_value(const v16i& acc, int g) {
 // a_n 2^32 (mod p_j), positive and below p 2^32, and one REDC gives a_n mod p_j.
 // Columns of the last k-step past 4L read the next coefficient's bytes; the
 // matching digits of D are zero.
-// |C| <= 4L 128^2 and L <= 127 (the plan's limit), so x = C0 + 256 C1 and
-// y = C2 + 256 C3 stay below 2^31 in magnitude, and the sum is
-// bias + x + 2^16 y with two 64-bit adds.
+// |C| <= 4L 128^2. Up to L = 127 (WIDE false), x = C0 + 256 C1 and y = C2 +
+// 256 C3 stay below 2^31 in magnitude, and the sum is bias + x + 2^16 y with
+// two 64-bit adds; wider points (WIDE) form x and y in 64 bits. Either way
+// |x + 2^16 y| <= L 2^40 < p 2^20 (the bias's offset) for L < 2^11, so the
+// REDC input is positive and below p 2^32.
 // ---------------------------------------------------------------------------
-template <int NCT>
+constexpr int kRnsNarrowL = 127;
+
+template <int NCT, bool WIDE = false>
 __device__ __forceinline__ void rns_tile(const DevPlan& D, const u32* __restrict__ a, u32* __restrict__ R, u32 n0,
                                          u32 count, int vec, u32* lds32, int tid, int nthreads) {
     constexpr int TN = 32 * NCT;
@@ -496,9 +519,15 @@ __device__ __forceinline__ void rns_tile(const DevPlan& D, const u32* __restrict
         return B;
     };
     auto reduce = [&](const v16i& acc, int g, u64 bias, u32 p, u32 pinv) {
-        const int x = (int)((u32)acc[4 * g] + ((u32)acc[4 * g + 1] << 8));
-        const int y = (int)((u32)acc[4 * g + 2] + ((u32)acc[4 * g + 3] << 8));
-        return redc64(bias + (u64)(long long)x + ((u64)(long long)y << 16), p, pinv);
+        if constexpr (WIDE) {
+            const long long x = (long long)acc[4 * g] + (long long)acc[4 * g + 1] * 256;
+            const long long y = (long long)acc[4 * g + 2] + (long long)acc[4 * g + 3] * 256;
+            return redc64(bias + (u64)x + ((u64)y << 16), p, pinv);
+        } else {
+            const int x = (int)((u32)acc[4 * g] + ((u32)acc[4 * g + 1] << 8));
+            const int y = (int)((u32)acc[4 * g + 2] + ((u32)acc[4 * g + 3] << 8));
+            return redc64(bias + (u64)(long long)x + ((u64)(long long)y << 16), p, pinv);
+        }
     };
     for (int rt = wave; rt < D.rns_rt; rt += nw) {
         v16i acc[NCT];
@@ -524,12 +553,12 @@ __device__ __forceinline__ void rns_tile(const DevPlan& D, const u32* __restrict
     }
 }
 
-template <int NCT>
+template <int NCT, bool WIDE>
 __global__ void __launch_bounds__(512) k_rns(DevPlan D, const u32* __restrict__ a0, const u32* __restrict__ a1,
                                               u32* __restrict__ R0, u32* __restrict__ R1, u32 count, int vec) {
     extern __shared__ __align__(16) u32 lds32[];
-    rns_tile<NCT>(D, blockIdx.y ? a1 : a0, blockIdx.y ? R1 : R0, blockIdx.x * (u32)(32 * NCT), count, vec, lds32,
-                  (int)threadIdx.x, (int)blockDim.x);
+    rns_tile<NCT, WIDE>(D, blockIdx.y ? a1 : a0, blockIdx.y ? R1 : R0, blockIdx.x * (u32)(32 * NCT), count, vec,
+                        lds32, (int)threadIdx.x, (int)blockDim.x);
 }
 
 // ---------------------------------------------------------------------------
@@ -742,29 +771,43 @@ __global__ void __launch_bounds__(256) k_crt_partial(DevPlan D, const u32* __res
 
 // Sums the shards' slots and finishes: c = CRT result mod q. D is the full
 // plan. The block first sums its coefficients' slots over the shards into LDS
-// (all threads, coalesced), then finishes one coefficient per thread.
-template <int NCT>
+// (all threads, coalesced), then finishes one coefficient per thread. DIRECT
+// (points whose slots do not fit the LDS beside the finish's limbs, W above
+// about 9,000 bits): the finishing threads read and sum the slots from global
+// memory as they go (consecutive threads, consecutive words).
+template <int NCT, bool DIRECT>
 __global__ void __launch_bounds__(256) k_crt_combine(DevPlan D, const long long* __restrict__ P, int shards,
                                                       size_t shard_stride, u32* __restrict__ out, u32 count) {
     extern __shared__ __align__(16) unsigned char lds8[];
     constexpr int TN = 32 * NCT;
     const int nslot = D.L + 3;
-    long long* S = reinterpret_cast<long long*>(lds8);                   // [nslot][TN]
-    u32* Z = reinterpret_cast<u32*>(lds8 + (size_t)nslot * TN * 8);      // [TN][L + 2]
     const u32 n0 = blockIdx.x * (u32)TN;
     const int cnt = (int)min((u32)TN, count - n0);
     const u32 N = D.N;
     const int tid = (int)threadIdx.x;
-    for (int idx = tid; idx < nslot * TN; idx += (int)blockDim.x) {
-        const int slot = idx / TN, c = idx % TN;
-        long long s = 0;
-        if (c < cnt)
-            for (int g = 0; g < shards; ++g) s += P[(size_t)g * shard_stride + (size_t)slot * N + n0 + c];
-        S[idx] = s;
+    if constexpr (DIRECT) {
+        u32* Z = reinterpret_cast<u32*>(lds8);  // [TN][L + 2]
+        crt_finish(D,
+                   [&](int slot, int c) {
+                       long long s = 0;
+                       for (int g = 0; g < shards; ++g) s += P[(size_t)g * shard_stride + (size_t)slot * N + n0 + c];
+                       return s;
+                   },
+                   out, n0, cnt, Z, tid, (int)blockDim.x);
+    } else {
+        long long* S = reinterpret_cast<long long*>(lds8);                   // [nslot][TN]
+        u32* Z = reinterpret_cast<u32*>(lds8 + (size_t)nslot * TN * 8);      // [TN][L + 2]
+        for (int idx = tid; idx < nslot * TN; idx += (int)blockDim.x) {
+            const int slot = idx / TN, c = idx % TN;
+            long long s = 0;
+            if (c < cnt)
+                for (int g = 0; g < shards; ++g) s += P[(size_t)g * shard_stride + (size_t)slot * N + n0 + c];
+            S[idx] = s;
+        }
+        __syncthreads();
+        crt_finish(D, [&](int slot, int c) { return S[(size_t)slot * TN + c]; }, out, n0, cnt, Z, tid,
+                   (int)blockDim.x);
     }
-    __syncthreads();
-    crt_finish(D, [&](int slot, int c) { return S[(size_t)slot * TN + c]; }, out, n0, cnt, Z, tid,
-               (int)blockDim.x);
 }
 
 template <int NCT>
@@ -775,8 +818,8 @@ __global__ void __launch_bounds__(256) k_crt(DevPlan D, const u32* __restrict__ 
 }
 
 // ---------------------------------------------------------------------------
-// Batched products: one launch per phase, each over that phase's tiles of
-// every product. Inside a phase tiles are independent, so there is no
+// Batched products (N <= 2^16, two NTT passes): one launch per phase, each
+// over that phase's tiles of every product. Inside a phase tiles are independent, so there is no
 // cross-block wait and no device-scope fence; the launch boundary orders the
 // phases. Each phase kernel has its own register budget (one kernel holding
 // every tile type would run at the largest one's). Two ways to hand out
@@ -787,14 +830,14 @@ template <bool INV>
 __device__ __forceinline__ void col_tile_nb(int nb, const DevPlan& D, u32* R, int j, int col0, int tc, u32* lds,
                                             int tid) {
     switch (nb) {
-        case 1: col_tile<1, INV>(D, R, j, col0, tc, lds, tid); break;
-        case 2: col_tile<2, INV>(D, R, j, col0, tc, lds, tid); break;
-        case 3: col_tile<3, INV>(D, R, j, col0, tc, lds, tid); break;
-        case 4: col_tile<4, INV>(D, R, j, col0, tc, lds, tid); break;
-        case 5: col_tile<5, INV>(D, R, j, col0, tc, lds, tid); break;
-        case 6: col_tile<6, INV>(D, R, j, col0, tc, lds, tid); break;
-        case 7: col_tile<7, INV>(D, R, j, col0, tc, lds, tid); break;
-        case 8: col_tile<8, INV>(D, R, j, col0, tc, lds, tid); break;
+        case 1: col_tile<1, INV>(D, R, j, col0, tc, lds, tid, 0, 0, D.w2); break;
+        case 2: col_tile<2, INV>(D, R, j, col0, tc, lds, tid, 0, 0, D.w2); break;
+        case 3: col_tile<3, INV>(D, R, j, col0, tc, lds, tid, 0, 0, D.w2); break;
+        case 4: col_tile<4, INV>(D, R, j, col0, tc, lds, tid, 0, 0, D.w2); break;
+        case 5: col_tile<5, INV>(D, R, j, col0, tc, lds, tid, 0, 0, D.w2); break;
+        case 6: col_tile<6, INV>(D, R, j, col0, tc, lds, tid, 0, 0, D.w2); break;
+        case 7: col_tile<7, INV>(D, R, j, col0, tc, lds, tid, 0, 0, D.w2); break;
+        case 8: col_tile<8, INV>(D, R, j, col0, tc, lds, tid, 0, 0, D.w2); break;
         default: break;
     }
 }
@@ -912,14 +955,14 @@ __global__ void __launch_bounds__(kBatchThreads[PH]) k_batch_phase(const BatchPr
 // ---------------------------------------------------------------------------
 template <bool INV>
 hipError_t col_dispatch(int nb, dim3 grid, dim3 block, size_t lds, hipStream_t s, const DevPlan& D, u32* R0, u32* R1,
-                        int tc) {
+                        int tc, int s0, int rest, int tiles_log) {
     switch (nb) {
-#define PM_COL(n)                                                                         \
-    case n:                                                                               \
-        if (tc == 32)                                                                     \
-            hipLaunchKernelGGL((k_col<n, INV, 32>), grid, block, lds, s, D, R0, R1, tc);  \
-        else                                                                              \
-            hipLaunchKernelGGL((k_col<n, INV, 0>), grid, block, lds, s, D, R0, R1, tc);   \
+#define PM_COL(n)                                                                                                    \
+    case n:                                                                                                          \
+        if (tc == 32)                                                                                                \
+            hipLaunchKernelGGL((k_col<n, INV, 32>), grid, block, lds, s, D, R0, R1, tc, s0, rest, tiles_log);        \
+        else                                                                                                         \
+            hipLaunchKernelGGL((k_col<n, INV, 0>), grid, block, lds, s, D, R0, R1, tc, s0, rest, tiles_log);         \
         break;
         PM_COL(1) PM_COL(2) PM_COL(3) PM_COL(4) PM_COL(5) PM_COL(6) PM_COL(7) PM_COL(8)
 #undef PM_COL
@@ -945,6 +988,26 @@ int line_threads(int nb) { return nb >= 4 ? 1 << (nb - 4) : 1; }
 
 }  // namespace
 
+// The finish over `count` coefficients of `shards` slot arrays (shard_stride
+// apart, stride N per slot), DIRECT when the slots do not fit the LDS.
+static hipError_t launch_combine(const DevPlan& D, const LaunchCfg& C, const long long* P, int shards,
+                                 size_t shard_stride, u32* out, u32 count, hipStream_t s) {
+    const dim3 grid((count + C.crt_tn - 1) / C.crt_tn), block(256);
+    const size_t lds = crt_combine_lds_bytes(C.crt_tn, D.L, C.combine_direct);
+    if (C.combine_direct) {
+        if (C.crt_tn != 32) return hipErrorInvalidValue;
+        hipLaunchKernelGGL((k_crt_combine<1, true>), grid, block, lds, s, D, P, shards, shard_stride, out, count);
+        return hipGetLastError();
+    }
+    switch (C.crt_tn) {
+        case 128: hipLaunchKernelGGL((k_crt_combine<4, false>), grid, block, lds, s, D, P, shards, shard_stride, out, count); break;
+        case 64: hipLaunchKernelGGL((k_crt_combine<2, false>), grid, block, lds, s, D, P, shards, shard_stride, out, count); break;
+        case 32: hipLaunchKernelGGL((k_crt_combine<1, false>), grid, block, lds, s, D, P, shards, shard_stride, out, count); break;
+        default: return hipErrorInvalidValue;
+    }
+    return hipGetLastError();
+}
+
 // PM_RNS_TN (32 .. 256) / PM_CRT_TN (32 .. 128) set the coefficients per
 // RNS / CRT block, for tuning; the LDS limit still applies.
 static int tile_cap(const char* name, int dflt, int maxv) {
@@ -953,11 +1016,11 @@ static int tile_cap(const char* name, int dflt, int maxv) {
     return (t == 32 || t == 64 || t == 128 || t == 256) && t <= maxv ? t : dflt;
 }
 
-LaunchCfg choose_launch(const DevPlan& D, size_t max_lds) {
+LaunchCfg choose_launch(const DevPlan& D, size_t max_lds, size_t device_lds) {
     LaunchCfg C;
-    const u32 nrows = 1u << D.w1, ncols = 1u << D.w2;
+    const u32 nrows = 1u << (D.logN - D.w2), ncols = 1u << D.w2;  // rows of the row pass, and their length
     C.rns_stride = 4 * D.L;
-    C.rns_tn = tile_cap("PM_RNS_TN", 128, 256);
+    C.rns_tn = D.L > kRnsNarrowL ? 32 : tile_cap("PM_RNS_TN", 128, 256);  // the 64-bit epilogue runs 32
     while (C.rns_tn > 32 && rns_lds_bytes(C.rns_tn, D.L, D.rns_rt) > max_lds) C.rns_tn /= 2;
     C.rns_lds = rns_lds_bytes(C.rns_tn, D.L, D.rns_rt);
     // 4-wave RNS blocks by default: four blocks per CU overlap their tile
@@ -971,6 +1034,14 @@ LaunchCfg choose_launch(const DevPlan& D, size_t max_lds) {
     auto crt_need = [&](int tn) { return crt_lds_bytes(tn, C.crt_ys, D.L, D.crt_ks); };
     while (C.crt_tn > 32 && crt_need(C.crt_tn) > max_lds) C.crt_tn /= 2;
     C.crt_lds = crt_need(C.crt_tn);
+    // Wide points whose CRT block exceeds the device's LDS split the CRT into
+    // the linear part (y and the GEMM, slots to global memory) and the finish.
+    C.crt_split = C.crt_lds > device_lds;
+    if (C.crt_split) {
+        C.crt_tn = 32;
+        C.crt_lds = crt_partial_lds_bytes(32, C.crt_ys, D.crt_ks);
+    }
+    C.combine_direct = crt_combine_lds_bytes(C.crt_tn, D.L, false) > device_lds;
     C.col_tc = (int)std::min<u32>(ncols, 32);
     C.row_rpb = (int)std::min<u32>(nrows, (u32)(256 / line_threads(D.w2)));
     return C;
@@ -989,28 +1060,46 @@ hipError_t launch_rns(const DevPlan& D, const LaunchCfg& C, const u32* a0, const
         R1 += first;
     }
     const int vec = ((uintptr_t)a0 % 16 == 0) && (nops < 2 || (uintptr_t)a1 % 16 == 0);
+    if (D.L > kRnsNarrowL) {  // 64-bit epilogue; such rows fill the LDS at 32 coefficients
+        if (C.rns_tn != 32) return hipErrorInvalidValue;
+        hipLaunchKernelGGL((k_rns<1, true>), grid, block, C.rns_lds, s, D, a0, a1, R0, R1, count, vec);
+        return hipGetLastError();
+    }
     switch (C.rns_tn) {
-        case 256: hipLaunchKernelGGL((k_rns<8>), grid, block, C.rns_lds, s, D, a0, a1, R0, R1, count, vec); break;
-        case 128: hipLaunchKernelGGL((k_rns<4>), grid, block, C.rns_lds, s, D, a0, a1, R0, R1, count, vec); break;
-        case 64: hipLaunchKernelGGL((k_rns<2>), grid, block, C.rns_lds, s, D, a0, a1, R0, R1, count, vec); break;
-        case 32: hipLaunchKernelGGL((k_rns<1>), grid, block, C.rns_lds, s, D, a0, a1, R0, R1, count, vec); break;
+        case 256: hipLaunchKernelGGL((k_rns<8, false>), grid, block, C.rns_lds, s, D, a0, a1, R0, R1, count, vec); break;
+        case 128: hipLaunchKernelGGL((k_rns<4, false>), grid, block, C.rns_lds, s, D, a0, a1, R0, R1, count, vec); break;
+        case 64: hipLaunchKernelGGL((k_rns<2, false>), grid, block, C.rns_lds, s, D, a0, a1, R0, R1, count, vec); break;
+        case 32: hipLaunchKernelGGL((k_rns<1, false>), grid, block, C.rns_lds, s, D, a0, a1, R0, R1, count, vec); break;
         default: return hipErrorInvalidValue;
     }
     return hipGetLastError();
 }
 
 hipError_t launch_col(const DevPlan& D, const LaunchCfg& C, u32* R0, u32* R1, int nops, bool inverse, hipStream_t s) {
-    if (D.w1 == 0) return hipSuccess;
-    const u32 ncols = 1u << D.w2;
-    const int T = line_threads(D.w1);
-    const dim3 grid(ncols / C.col_tc, D.K, nops), block(C.col_tc * T);
-    const size_t lds = col_lds_bytes(D.w1, C.col_tc);
-    return inverse ? col_dispatch<true>(D.w1, grid, block, lds, s, D, R0, R1, C.col_tc)
-                   : col_dispatch<false>(D.w1, grid, block, lds, s, D, R0, R1, C.col_tc);
+    // the strided passes 0 .. npass - 2 (forward in order, inverse in reverse)
+    const int last = D.npass - 1;
+    for (int k = 0; k < last; ++k) {
+        const int pi = inverse ? last - 1 - k : k;
+        const int nb = D.pw[pi];
+        if (nb == 0) continue;
+        int s0 = 0;
+        for (int i = 0; i < pi; ++i) s0 += D.pw[i];
+        const int rest = D.logN - s0 - nb;
+        const int tc = (int)std::min<u32>((u32)C.col_tc, 1u << rest);
+        int tiles_log = 0;
+        while ((tc << tiles_log) < (1 << rest)) ++tiles_log;
+        const int T = line_threads(nb);
+        const dim3 grid((1u << s0) << tiles_log, D.K, nops), block(tc * T);
+        const size_t lds = col_lds_bytes(nb, tc);
+        const hipError_t e = inverse ? col_dispatch<true>(nb, grid, block, lds, s, D, R0, R1, tc, s0, rest, tiles_log)
+                                     : col_dispatch<false>(nb, grid, block, lds, s, D, R0, R1, tc, s0, rest, tiles_log);
+        if (e != hipSuccess) return e;
+    }
+    return hipSuccess;
 }
 
 hipError_t launch_row(const DevPlan& D, const LaunchCfg& C, u32* RB, const u32* RA, RowMode mode, hipStream_t s) {
-    const u32 nrows = 1u << D.w1;
+    const u32 nrows = 1u << (D.logN - D.w2);
     const int T = line_threads(D.w2);
     const dim3 grid(nrows / C.row_rpb, D.K), block(C.row_rpb * T);
     const size_t rs = ((size_t)1 << D.w2) + (D.w2 > 4 ? 16 : 0);
@@ -1024,10 +1113,18 @@ hipError_t launch_row(const DevPlan& D, const LaunchCfg& C, u32* RB, const u32* 
 }
 
 hipError_t launch_crt(const DevPlan& D, const LaunchCfg& C, const u32* R, u32* out, const uint2* G, hipStream_t s,
-                      u32 first, u32 count) {
+                      u32 first, u32 count, long long* part) {
     if (count == 0) count = D.N - first;
     if (first + count > D.N) return hipErrorInvalidValue;
     const dim3 grid((count + C.crt_tn - 1) / C.crt_tn), block(256);
+    if (C.crt_split) {
+        // slots of [first, first + count) into part (stride N per slot), then the finish
+        if (!part) return hipErrorInvalidValue;
+        hipLaunchKernelGGL((k_crt_partial<1>), grid, block, C.crt_lds, s, D, R + first, G, part + first, count,
+                           C.crt_ys);
+        if (const hipError_t e = hipGetLastError(); e != hipSuccess) return e;
+        return launch_combine(D, C, part + first, 1, 0, out + (size_t)first * D.L, count, s);
+    }
     R += first;
     out += (size_t)first * D.L;
     switch (C.crt_tn) {
@@ -1041,7 +1138,7 @@ hipError_t launch_crt(const DevPlan& D, const LaunchCfg& C, const u32* R, u32* o
 
 hipError_t launch_crt_partial(const DevPlan& D, const LaunchCfg& C, const u32* R, long long* P, hipStream_t s) {
     const dim3 grid((D.N + C.crt_tn - 1) / C.crt_tn), block(256);
-    const size_t lds = (size_t)C.crt_tn * C.crt_ys + (size_t)128 * D.crt_ks;
+    const size_t lds = crt_partial_lds_bytes(C.crt_tn, C.crt_ys, D.crt_ks);
     switch (C.crt_tn) {
         case 128: hipLaunchKernelGGL((k_crt_partial<4>), grid, block, lds, s, D, R, D.G, P, D.N, C.crt_ys); break;
         case 64: hipLaunchKernelGGL((k_crt_partial<2>), grid, block, lds, s, D, R, D.G, P, D.N, C.crt_ys); break;
@@ -1053,15 +1150,7 @@ hipError_t launch_crt_partial(const DevPlan& D, const LaunchCfg& C, const u32* R
 
 hipError_t launch_crt_combine(const DevPlan& D, const LaunchCfg& C, const long long* P, int shards, size_t shard_stride,
                               u32* out, hipStream_t s) {
-    const dim3 grid((D.N + C.crt_tn - 1) / C.crt_tn), block(256);
-    const size_t lds = (size_t)C.crt_tn * ((D.L + 3) * 8 + (D.L + 2) * 4);
-    switch (C.crt_tn) {
-        case 128: hipLaunchKernelGGL((k_crt_combine<4>), grid, block, lds, s, D, P, shards, shard_stride, out, D.N); break;
-        case 64: hipLaunchKernelGGL((k_crt_combine<2>), grid, block, lds, s, D, P, shards, shard_stride, out, D.N); break;
-        case 32: hipLaunchKernelGGL((k_crt_combine<1>), grid, block, lds, s, D, P, shards, shard_stride, out, D.N); break;
-        default: return hipErrorInvalidValue;
-    }
-    return hipGetLastError();
+    return launch_combine(D, C, P, shards, shard_stride, out, D.N, s);
 }
 
 hipError_t launch_copy_probe(const void* src, void* dst, size_t bytes, int grid, hipStream_t s) {

@@ -309,11 +309,11 @@ void State::run_host(const uint32_t* a, const uint32_t* b, uint32_t* c) {
     // would pace the copies). Each D2H waits for the launch that covers it.
     const int nout = (int)P.cout.size();
     const uint32_t first_n = P.cout[0].count;
-    hip_check(launch_crt(dev, cfg, d_R[1], d_out, dev.G, stream, 0, first_n), "crt");
+    hip_check(launch_crt(dev, cfg, d_R[1], d_out, dev.G, stream, 0, first_n, d_slots), "crt");
     hip_check(hipEventRecord(P.ev_crt[0], stream), "record");
     if (P.profile) hip_check(hipEventRecord(P.tl_more[1], stream), "record");
     if (nout > 1) {
-        hip_check(launch_crt(dev, cfg, d_R[1], d_out, dev.G, stream, first_n, host.N - first_n), "crt");
+        hip_check(launch_crt(dev, cfg, d_R[1], d_out, dev.G, stream, first_n, host.N - first_n, d_slots), "crt");
         hip_check(hipEventRecord(P.ev_crt[1], stream), "record");
     }
     release_R(stream);

@@ -9,16 +9,23 @@
 
 namespace pm {
 
-// Limits: N above 2^16 would need a third NTT pass.
-constexpr int kMaxLogN = 16;
-// The RNS epilogue sums its 4L-term digit products in 32 bits two at a time
-// (4L 128^2 257 < 2^31), which holds up to L = 127, i.e. W <= 4064.
-constexpr uint32_t kMaxL = 127;
+// Limits. N: primes p = 1 (mod 2N) below 2^31 exist only for 2N < 2^31, and
+// four NTT passes of at most 8 stages cover 2^30. K < 1024, as the reference
+// (solve.cu:911-913), which bounds W (about 15,800 bits at small N).
+constexpr int kMaxLogN = 30;
 constexpr int kMaxK = 1023;
+// The RNS epilogue sums its 4L-term digit products in 32 bits two at a time
+// (4L 128^2 257 < 2^31) up to L = 127, W <= 4064; wider points take the 64-bit
+// epilogue. The batched and served forms keep to L <= kNarrowL.
+constexpr uint32_t kNarrowL = 127;
 
 struct HostPlan {
     uint32_t N = 0, W = 0, L = 0;
-    int logN = 0, w1 = 0, w2 = 0;  // 4-step split: nrows = 2^w1, ncols = 2^w2
+    int logN = 0, w1 = 0, w2 = 0;  // 4-step split: nrows = 2^w1, ncols = 2^w2 (two passes)
+    // NTT passes of pw[i] stages each, the last the row pass (w1 = pw[0], w2 =
+    // pw[npass-1]): two up to N = 2^16, then ceil(logN / 8).
+    int npass = 2;
+    int pw[4] = {0, 0, 0, 0};
     int K = 0;                      // RNS primes
     uint64_t delta = 0;             // 2^W - q
     std::vector<uint32_t> q;        // [L]

@@ -30,6 +30,10 @@ struct State {
     size_t device_bytes = 0;
     Pipeline* pipe = nullptr;
     long long* d_part = nullptr;  // prime-shard CRT partials, grown on demand
+    // Points with the split CRT (cfg.crt_split): its int64 slots, (L + 3) N,
+    // for the default residues (d_slots) and for each lane.
+    long long* d_slots = nullptr;
+    size_t slot_bytes() const { return (size_t)(host.L + 3) * host.N * 8; }
     // This device's scratch array (the allocation-free call's residues), or
     // nullptr and the lookup's error.
     uint32_t* d_scratch = nullptr;
@@ -49,7 +53,7 @@ struct State {
     // The whole multiply on device buffers, enqueued on st.
     hipError_t enqueue_multiply(const uint32_t* a, const uint32_t* b, uint32_t* c, hipStream_t st);
     hipError_t enqueue_multiply_with(const uint32_t* a, const uint32_t* b, uint32_t* c, hipStream_t st,
-                                     uint32_t* R0, uint32_t* R1);
+                                     uint32_t* R0, uint32_t* R1, long long* slots);
 
     // Device calls without a caller stream rotate over `lanes` lanes (lane 0
     // is the state's stream; each lane has its own stream and residues), so
@@ -61,6 +65,7 @@ struct State {
     int lanes = 3;
     hipStream_t lane_stream[kMaxLanes] = {};
     uint32_t* lane_R[kMaxLanes][2] = {};
+    long long* lane_slots[kMaxLanes] = {};
     hipEvent_t ev_crt[kMaxLanes] = {};
     int prev_lane = -1;  // lane of the last call, -1 before the first
     int next_lane = 0;

@@ -50,24 +50,41 @@ def test_point_info_matches_oracle(O, pm, N, W):
     ((64, 32, 1, "even"), "q out of range"),
     ((64, 32, 1, "tiny"), "too far below"),
     ((64, 64, 2, "toplimb"), "top limb is zero"),
-    ((131072, 32, 1, "q"), "not supported"),
+    ((1 << 26, 4, 1, "q13"), "not enough NTT primes"),
+    ((1, 15900, 497, "wide"), "too many RNS primes"),
 ])
 def test_point_errors(O, pm, args, msg):
+    """The reference's refusals (solve.cu:873-913), and only those: N = 2^26
+    has a single prime 1 mod 2^27 below 2^31, and W = 15900 needs more than
+    1023 primes."""
     N, W, L, which = args
     q = O.spec_q(64, 32)
     qv = {"q": np.array([q], np.uint32), "q2": np.array([q, 0], np.uint32),
           "even": np.array([q - 1], np.uint32), "tiny": np.array([3], np.uint32),
-          "toplimb": np.array([q, 0], np.uint32)}[which]
+          "toplimb": np.array([q, 0], np.uint32), "q13": np.array([13], np.uint32),
+          "wide": O.q_limbs((1 << 15900) - 3, 497)}[which]
     with pytest.raises(pm.PolymultError, match=msg):
         pm.point_info(N, W, L, qv)
 
 
-def test_width_limit(pm):
-    """L above 127 is refused before any table is built (the RNS epilogue's
-    32-bit digit sums are exact up to there)."""
-    q = np.full(128, 0xFFFFFFFF, np.uint32)
-    with pytest.raises(pm.PolymultError, match="W above 4064"):
-        pm.point_info(512, 4065, 128, q)
+@pytest.mark.parametrize("N,W,delta", [(131072, 64, 0), (262144, 32, 0), (1 << 20, 16, 15), (1 << 25, 8, 5),
+                                       (512, 4065, 1000001), (64, 5000, 1000001), (16, 8192, 1000001),
+                                       (1, 15840, 3)])
+def test_points_beyond_two_passes_and_4064_bits(O, pm, N, W, delta):
+    """Points the reference accepts past N = 65536 (three and four NTT passes)
+    and past W = 4064 (the 64-bit RNS epilogue, the split CRT): pm_point_info
+    takes them, with the oracle's K and primes (delta 0: the spec's prime q;
+    else q = 2^W - delta, any odd modulus the reference accepts)."""
+    q = O.spec_q(N, W) if delta == 0 else (1 << W) - delta
+    L = O.limbs_of(W)
+    K, d, primes = pm.point_info(N, W, L, O.q_limbs(q, L))
+    assert d == (1 << W) - q
+    if N * K <= 1 << 22:
+        P = O.OraclePlan(N, W, q)
+        assert K == P.K and np.array_equal(primes, P.primes)
+    else:
+        # 2^25: the two primes 1 mod 2^26 below 2^31, largest first
+        assert list(primes) == [2013265921, 1811939329]
 
 
 def test_q_limb_count_checked(O, pm):
