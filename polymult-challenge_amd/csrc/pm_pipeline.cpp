@@ -2,15 +2,18 @@
 //
 // The call moves 2 N L 4 bytes in and N L 4 bytes out, so everything else is
 // arranged to hide behind those transfers:
-//   * a and b are cut into contiguous coefficient chunks. A host worker team
-//     copies chunk i from the caller's pageable buffer into pinned staging
-//     (non-temporal stores) while the copy engine moves chunk i-1 to HBM and
-//     the RNS kernel decomposes chunk i-2 on the compute stream;
-//   * a's forward column pass runs once a's chunks are in (while b's are
-//     still arriving); once every chunk is resident the other passes run;
-//   * the result leaves in chunks: CRT of chunk i on the compute stream, its
-//     device-to-host copy on a second copy stream, and the host team scatters
-//     chunk i-1 from pinned staging into the caller's buffer.
+//   * a is cut into growing contiguous coefficient chunks (1, 3, then 16 MiB)
+//     and b goes as one copy: a host worker team stages a chunk from the
+//     caller's pageable buffer into pinned memory (non-temporal stores) at
+//     about 4x the link rate while the copy engine moves the one before and
+//     the RNS kernel decomposes the one before that; the engines run fastest
+//     on large copies, and all of b is staged during a's last copy;
+//   * a's forward column pass runs once a is in (while b crosses the link);
+//     once b is resident the other passes run;
+//   * the result leaves in chunks over two copy streams: CRT of the first
+//     chunk alone, then the rest, device-to-host copies alternating between
+//     the streams, and the host team scatters each chunk from pinned staging
+//     into the caller's buffer while the next is copied.
 // Chunks are contiguous coefficient ranges (the input layout is [N][L]), so
 // staging needs no gather; the column NTT pass simply starts after the last
 // input chunk instead of per chunk.
@@ -99,13 +102,14 @@ int host_cpus() {
 struct Pipeline {
     Team* team = nullptr;
     hipStream_t s_in = nullptr, s_out = nullptr;
-    hipStream_t s_in2 = nullptr;  // second H2D stream: input chunks alternate
+    hipStream_t s_in2 = nullptr;   // second H2D stream: input chunks alternate
+    hipStream_t s_out2 = nullptr;  // optional second D2H stream (PM_D2H_STREAMS=2)
     uint32_t* h_in[2] = {nullptr, nullptr};  // pinned staging, [N][L]
     uint32_t* h_out = nullptr;
     struct Chunk {
         uint32_t first, count;
     };
-    std::vector<Chunk> cin, cout;
+    std::vector<Chunk> cin[2], cout;  // input plans of a and b, output plan
     std::vector<hipEvent_t> ev_in, ev_crt, ev_out;
     bool profile = false;
     // PM_PROFILE timeline: start of input, last H2D done, last RNS done,
@@ -154,10 +158,11 @@ std::vector<Pipeline::Chunk> plan_chunks(uint32_t N, size_t bytes_per_coeff, con
     return out;
 }
 
-// "0.5,2,8" (MiB) from the environment, or the default.
+// "0.5,2,8" (MiB) from the environment ("0": none), or the default.
 std::vector<size_t> sizes_from_env(const char* name, std::vector<size_t> dflt) {
     const char* e = std::getenv(name);
     if (!e || !*e) return dflt;
+    if (std::strcmp(e, "0") == 0) return {};
     std::vector<size_t> out;
     for (const char* p = e; *p;) {
         char* end = nullptr;
@@ -194,18 +199,27 @@ void State::pipeline_init() {
     const char* hs = std::getenv("PM_H2D_STREAMS");
     if (!hs || std::atoi(hs) != 1) hip_check(hipStreamCreateWithFlags(&P->s_in2, hipStreamNonBlocking), "stream");
     hip_check(hipStreamCreateWithFlags(&P->s_out, hipStreamNonBlocking), "stream");
+    // Output chunks alternate over two D2H streams (2-3% faster per call at
+    // N=65536/W=1747 than one); PM_D2H_STREAMS=1 keeps one.
+    const char* ds = std::getenv("PM_D2H_STREAMS");
+    if (!ds || std::atoi(ds) != 1) hip_check(hipStreamCreateWithFlags(&P->s_out2, hipStreamNonBlocking), "stream");
     const size_t bytes = coeff_words * 4;
     for (int i = 0; i < 2; ++i) hip_check(hipHostMalloc((void**)&P->h_in[i], bytes, hipHostMallocDefault), "pinned");
     hip_check(hipHostMalloc((void**)&P->h_out, bytes, hipHostMallocDefault), "pinned");
     const size_t bpc = (size_t)host.L * 4;
     const size_t MiB = 1u << 20;
-    // Defaults from sweeps at N=65536/W=1747. The input plans measured within
-    // 40 us of each other (the copy engine sets the pace). On the output side,
-    // 5 MiB chunks after the first let the scatter into the caller's buffer
-    // run under the next chunk's copy: 1.08-1.11 ms per call against
-    // 1.13-1.18 ms with one 14 MiB chunk.
-    P->cin = plan_chunks(host.N, bpc, sizes_from_env("PM_IN_CHUNKS", {MiB, 4 * MiB, 8 * MiB}),
-                         sizes_from_env("PM_IN_TAIL", {MiB}), 128);
+    // Defaults from sweeps at N=65536/W=1747. The copy engine sets the pace:
+    // 1 MiB copies move at 40 GB/s, 8 MiB at 54, 64 MiB at 57. a starts small
+    // (the engine starts early) and grows; b, staged in full while a's last
+    // copy runs, goes as one copy. Against {1, 4, 8} MiB plus a 1 MiB tail
+    // for both operands: 1.031-1.036 ms per call against 1.078-1.093 (with
+    // the two D2H streams). On the output side, 5 MiB chunks after the first
+    // let the scatter into the caller's buffer run under the next chunk's
+    // copy: 1.08-1.11 ms per call against 1.13-1.18 ms with one 14 MiB chunk.
+    P->cin[0] = plan_chunks(host.N, bpc, sizes_from_env("PM_IN_CHUNKS", {MiB, 3 * MiB, 16 * MiB}),
+                            sizes_from_env("PM_IN_TAIL", {}), 128);
+    P->cin[1] = plan_chunks(host.N, bpc, sizes_from_env("PM_IN_CHUNKS_B", {64 * MiB}),
+                            sizes_from_env("PM_IN_TAIL_B", {}), 128);
     P->cout = plan_chunks(host.N, bpc, sizes_from_env("PM_OUT_CHUNKS", {MiB, 5 * MiB}),
                           sizes_from_env("PM_OUT_TAIL", {MiB / 2}), 128);
     if (P->profile) {
@@ -216,12 +230,12 @@ void State::pipeline_init() {
         v.resize(n);
         for (auto& e : v) hip_check(hipEventCreateWithFlags(&e, hipEventDisableTiming), "event");
     };
-    mk(P->ev_in, 2 * P->cin.size());
+    mk(P->ev_in, P->cin[0].size() + P->cin[1].size());
     mk(P->ev_crt, P->cout.size());
     mk(P->ev_out, P->cout.size());
     if (P->profile)
         std::fprintf(stderr, "pm: N=%u W=%u L=%u K=%d, w1=%d w2=%d, chunks in %zu x2 / out %zu, %d host threads\n",
-                     host.N, host.W, host.L, host.K, host.w1, host.w2, P->cin.size(), P->cout.size(),
+                     host.N, host.W, host.L, host.K, host.w1, host.w2, P->cin[0].size(), P->cout.size(),
                      P->team->size());
 }
 
@@ -231,6 +245,7 @@ void State::pipeline_free() {
     if (P->s_in) (void)hipStreamSynchronize(P->s_in);
     if (P->s_in2) (void)hipStreamSynchronize(P->s_in2);
     if (P->s_out) (void)hipStreamSynchronize(P->s_out);
+    if (P->s_out2) (void)hipStreamSynchronize(P->s_out2);
     for (auto e : P->ev_in) (void)hipEventDestroy(e);
     for (auto e : P->ev_crt) (void)hipEventDestroy(e);
     for (auto e : P->ev_out) (void)hipEventDestroy(e);
@@ -244,6 +259,7 @@ void State::pipeline_free() {
     if (P->s_in) (void)hipStreamDestroy(P->s_in);
     if (P->s_in2) (void)hipStreamDestroy(P->s_in2);
     if (P->s_out) (void)hipStreamDestroy(P->s_out);
+    if (P->s_out2) (void)hipStreamDestroy(P->s_out2);
     delete P->team;
     delete P;
     pipe = nullptr;
@@ -273,14 +289,13 @@ void State::run_host(const uint32_t* a, const uint32_t* b, uint32_t* c) {
 
     // ---- input: stage, H2D and RNS per chunk ----
     acquire_R(stream);  // d_R: after any caller-stream or sharded call still in flight
-    const int nin = (int)P.cin.size();
     if (P.profile) hip_check(hipEventRecord(P.tl[0], P.s_in), "record");
+    int i = -1;  // running input chunk index over both operands
     for (int op = 0; op < 2; ++op) {
-        for (int k = 0; k < nin; ++k) {
-            const auto ch = P.cin[k];
+        for (const auto ch : P.cin[op]) {
+            ++i;
             const size_t off = (size_t)ch.first * L, words = (size_t)ch.count * L;
             copy_job(P.h_in[op] + off, src[op] + off, words);
-            const int i = op * nin + k;
             hipStream_t sc = (P.s_in2 && (i & 1)) ? P.s_in2 : P.s_in;
             hip_check(hipMemcpyAsync(d_in[op] + off, P.h_in[op] + off, words * 4, hipMemcpyHostToDevice, sc), "H2D");
             hip_check(hipEventRecord(P.ev_in[i], sc), "record");
@@ -292,7 +307,7 @@ void State::run_host(const uint32_t* a, const uint32_t* b, uint32_t* c) {
     }
     const auto t_in = clock::now();
     if (P.profile) {
-        if (P.s_in2) hip_check(hipStreamWaitEvent(P.s_in, P.ev_in[2 * nin - 1], 0), "wait");
+        if (P.s_in2) hip_check(hipStreamWaitEvent(P.s_in, P.ev_in[i], 0), "wait");
         hip_check(hipEventRecord(P.tl[1], P.s_in), "record");
         hip_check(hipEventRecord(P.tl[2], stream), "record");
     }
@@ -320,12 +335,16 @@ void State::run_host(const uint32_t* a, const uint32_t* b, uint32_t* c) {
     for (int k = 0; k < nout; ++k) {
         const auto ch = P.cout[k];
         const size_t off = (size_t)ch.first * L, words = (size_t)ch.count * L;
-        if (k < 2) hip_check(hipStreamWaitEvent(P.s_out, P.ev_crt[k], 0), "wait");
-        hip_check(hipMemcpyAsync(P.h_out + off, d_out + off, words * 4, hipMemcpyDeviceToHost, P.s_out), "D2H");
-        hip_check(hipEventRecord(P.ev_out[k], P.s_out), "record");
-        if (P.profile && k == 0) hip_check(hipEventRecord(P.tl_more[2], P.s_out), "record");
+        hipStream_t so = (P.s_out2 && (k & 1)) ? P.s_out2 : P.s_out;
+        if (k < 2 || so != P.s_out) hip_check(hipStreamWaitEvent(so, P.ev_crt[std::min(k, 1)], 0), "wait");
+        hip_check(hipMemcpyAsync(P.h_out + off, d_out + off, words * 4, hipMemcpyDeviceToHost, so), "D2H");
+        hip_check(hipEventRecord(P.ev_out[k], so), "record");
+        if (P.profile && k == 0) hip_check(hipEventRecord(P.tl_more[2], so), "record");
     }
-    if (P.profile) hip_check(hipEventRecord(P.tl_more[3], P.s_out), "record");
+    if (P.profile) {
+        if (P.s_out2) hip_check(hipStreamWaitEvent(P.s_out, P.ev_out[nout - 1], 0), "wait");
+        hip_check(hipEventRecord(P.tl_more[3], P.s_out), "record");
+    }
     clock::time_point t_first{};
     for (int k = 0; k < nout; ++k) {
         wait_event(P.ev_out[k]);
@@ -338,6 +357,7 @@ void State::run_host(const uint32_t* a, const uint32_t* b, uint32_t* c) {
         auto us = [&](clock::time_point t) { return std::chrono::duration<double, std::micro>(t - t0).count(); };
         std::fprintf(stderr, "pm: inputs staged+submitted %.1f us | first out chunk %.1f | done %.1f\n", us(t_in),
                      us(t_first), us(clock::now()));
+        hip_check(hipEventSynchronize(P.tl_more[3]), "sync");  // recorded after the last D2H's own event
         auto el = [&](hipEvent_t e) {
             float ms = 0;
             hip_check(hipEventElapsedTime(&ms, P.tl[0], e), "elapsed");

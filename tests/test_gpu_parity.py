@@ -359,16 +359,17 @@ def test_distinct_states_on_threads(O, pm):
 
 
 @pytest.mark.parametrize("env", [
-    {"PM_IN_CHUNKS": "0.05,0.2", "PM_IN_TAIL": "0.03,0.01", "PM_OUT_CHUNKS": "0.1", "PM_OUT_TAIL": "0.02"},
-    {"PM_IN_CHUNKS": "500", "PM_IN_TAIL": "400", "PM_OUT_CHUNKS": "500", "PM_OUT_TAIL": "400"},
-    {"PM_HOST_THREADS": "1", "PM_H2D_STREAMS": "1"},
-    {"PM_HOST_THREADS": "3", "PM_IN_CHUNKS": "1,0.3", "PM_OUT_CHUNKS": "0.7,2"},
+    {"PM_IN_CHUNKS": "0.05,0.2", "PM_IN_TAIL": "0.03,0.01", "PM_IN_CHUNKS_B": "0.07", "PM_IN_TAIL_B": "0.02",
+     "PM_OUT_CHUNKS": "0.1", "PM_OUT_TAIL": "0.02"},
+    {"PM_IN_CHUNKS": "500", "PM_IN_TAIL": "400", "PM_IN_TAIL_B": "400", "PM_OUT_CHUNKS": "500", "PM_OUT_TAIL": "400"},
+    {"PM_HOST_THREADS": "1", "PM_H2D_STREAMS": "1", "PM_D2H_STREAMS": "1"},
+    {"PM_HOST_THREADS": "3", "PM_IN_CHUNKS": "1,0.3", "PM_IN_CHUNKS_B": "0.5,2", "PM_OUT_CHUNKS": "0.7,2"},
 ])
 def test_host_pipeline_plans(O, pm, env, monkeypatch):
-    """pm_run (the fherma_run path) under other chunk plans, host-thread
-    counts and one H2D stream: dozens of tiny chunks, one chunk larger than
-    the operand, tails larger than N. The plans are read when a state is
-    created."""
+    """pm_run (the fherma_run path) under other chunk plans (a's and b's
+    apart), host-thread counts and one copy stream each way: dozens of tiny
+    chunks, one chunk larger than the operand, tails larger than N. The plans
+    are read when a state is created."""
     for k, v in env.items():
         monkeypatch.setenv(k, v)
     print("pipeline plan %s" % env, flush=True)
