@@ -622,7 +622,9 @@ int pm_probe_copy(size_t bytes, int iters, double* gbps) {
         int dev = 0, cus = 0;
         hip_check(hipGetDevice(&dev), "hipGetDevice");
         hip_check(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev), "attribute");
-        const int grid = cus * 8;
+        // 4 blocks of 256 threads per CU: the fastest of 4, 8, 16 and 32 per CU
+        // over 1 GiB (5.66 TB/s against 4.55-4.78 TB/s for 8 and more)
+        const int grid = cus * 4;
         try {
             hip_check(hipMalloc(&src, bytes), "hipMalloc");
             hip_check(hipMalloc(&dst, bytes), "hipMalloc");
