@@ -11,8 +11,8 @@ fherma_run replacement) is reported beside it, never as `value`.
     python bench.py [--gpus 1] [--steps 2000] [--warmup 200] [--no-cpu] [--serve-seconds 30] [--no-serve]
                     [--legs plain,points,shard,e2e,batch,serve,cpu] [--copy-probe BYTES]
 
-Before the timed steps the multiply runs for at least SETTLE_S seconds (and
---warmup steps), so the clocks have settled whatever --warmup is; the timed
+Before the timed steps the multiply runs for at least SETTLE_S (1 s) and
+--warmup steps, so the clocks have settled whatever --warmup is; the timed
 steps are exactly --steps.
 
 Prints one JSON line.
@@ -46,7 +46,10 @@ PCIE_PEAK_GBS = 64.0  # PCIe Gen5 x16, per direction (vendor figure)
 VALU_INT32_OPS = 1024 * 16 * 2.4e9
 BUTTERFLY_VALU = 12
 PEAK_MODMUL_PER_S = VALU_INT32_OPS / BUTTERFLY_VALU
-SETTLE_S = 0.3  # time-based warm-up before the timed steps (clocks settle)
+# Time-based warm-up before the timed steps, so the clocks have settled
+# whatever --warmup is: after 0.3 s the per-kernel times still read 3-4%
+# above those of a run that had done 0.5 s more work.
+SETTLE_S = 1.0
 KERNEL_TIMING_LAUNCHES = 100  # launches per kernel in the HIP-event kernel times
 METRIC = "mults/s at N=65536/W=1747 (plain vs batched), %HBM/PCIe peak, serve p99 ms"
 KERNELS = ["rns", "col_fwd", "row_fused", "col_inv", "crt"]

@@ -234,8 +234,8 @@ void State::pipeline_init() {
     mk(P->ev_crt, P->cout.size());
     mk(P->ev_out, P->cout.size());
     if (P->profile)
-        std::fprintf(stderr, "pm: N=%u W=%u L=%u K=%d, w1=%d w2=%d, chunks in %zu x2 / out %zu, %d host threads\n",
-                     host.N, host.W, host.L, host.K, host.w1, host.w2, P->cin[0].size(), P->cout.size(),
+        std::fprintf(stderr, "pm: N=%u W=%u L=%u K=%d, w1=%d w2=%d, chunks in a %zu / b %zu / out %zu, %d host threads\n",
+                     host.N, host.W, host.L, host.K, host.w1, host.w2, P->cin[0].size(), P->cin[1].size(), P->cout.size(),
                      P->team->size());
 }
 
