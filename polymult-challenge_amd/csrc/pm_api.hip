@@ -268,11 +268,12 @@ void State::wait_for_inputs(hipStream_t st, int lane, const uint32_t* a, const u
 }
 
 void State::note_output(std::vector<Range>& list, hipEvent_t ev_before, const uint32_t* c) {
+    const Range r{(uintptr_t)c, (uintptr_t)c + coeff_words * 4};
+    if (std::find(list.begin(), list.end(), r) != list.end()) return;  // the usual case: one output buffer reused
     // ev_before still marks the end of the previous call that this list
     // covers: once it has completed, none of the listed outputs is pending.
     if (!list.empty() && hipEventQuery(ev_before) == hipSuccess) list.clear();
-    const Range r{(uintptr_t)c, (uintptr_t)c + coeff_words * 4};
-    if (std::find(list.begin(), list.end(), r) == list.end()) list.push_back(r);
+    list.push_back(r);
 }
 
 void State::sync_all() {
