@@ -115,6 +115,24 @@ def test_batch_closed_form_full_size(O, pm):
         assert O.unpack(dc[i].download((N, L))) == want, i
 
 
+def test_batch_far_moduli(O, pm):
+    """Products whose q = 2^W - delta with delta at the largest the point check
+    takes (below 2^63 and below 2^(W-1)) beside a spec-prime product, in one
+    batch: each equals the oracle's."""
+    far = (1 << 63) - 1
+    prods = [(256, 64, (1 << 64) - far), (1024, 100, (1 << 100) - far), (16, 33, (1 << 33) - ((1 << 32) - 1)),
+             (4096, 256, O.spec_q(4096, 256)), (64, 4064, (1 << 4064) - far), (65536, 1747, (1 << 1747) - far)]
+    ops = [O.operands(k, N, W, q, 6400 + i)
+           for i, (k, (N, W, q)) in enumerate(zip(["max", "uniform", "mixed", "uniform", "mixed", "mixed"], prods))]
+    B = pm.Batch(prods)
+    da = [pm.DeviceBuffer.from_array(a) for a, _ in ops]
+    db = [pm.DeviceBuffer.from_array(b) for _, b in ops]
+    dc = [pm.DeviceBuffer(N * O.limbs_of(W) * 4) for N, W, _ in prods]
+    B.run(da, db, dc)
+    B.sync()
+    _check(O, prods, ops, dc)
+
+
 def test_batch_widest_point(O, pm):
     """W = 4064 (L = 127, K = 263 of the shared primes) beside narrow
     products, in one group and as single-product groups: every product

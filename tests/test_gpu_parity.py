@@ -118,6 +118,40 @@ def test_multiply_widest_point(O, pm):
             assert np.array_equal(got, want), "%s: %d of %d coefficients differ" % (kind, _diff(got, want), N)
 
 
+# q = 2^W - delta with delta at the largest the point check takes (below 2^63
+# and below 2^(W-1), solve.cu:883-896): the CRT's 2^W = delta fold then
+# carries the most bits per round and runs the most rounds.
+FAR_MODULI = [(64, 2, 1), (16, 33, (1 << 32) - 1), (256, 64, (1 << 63) - 1), (1024, 100, (1 << 63) - 1),
+              (4096, 1000, (1 << 63) - 1), (64, 4064, (1 << 63) - 1), (8, 5000, (1 << 63) - 1),
+              (65536, 1747, (1 << 63) - 1)]
+
+
+@pytest.mark.parametrize("N,W,delta", FAR_MODULI)
+def test_multiply_far_modulus(O, pm, N, W, delta):
+    """Moduli far below 2^W, through the host and the device path: all-(q-1)
+    operands against the closed form c_k = (2k + 2 - N) mod q (any odd q:
+    (q-1)^2 = 1), the rest against the oracle."""
+    q = (1 << W) - delta
+    L = O.limbs_of(W)
+    P = O.OraclePlan(N, W, q)
+    with pm.Multiplier(N, W, O.q_limbs(q, L)) as m:
+        print("multiply N=%d W=%d K=%d delta bits %d" % (N, W, m.K, delta.bit_length()), flush=True)
+        assert m.K == P.K
+        am = np.tile(O.q_limbs(q - 1, L), (N, 1))
+        want = np.stack([O.q_limbs((2 * k + 2 - N) % q, L) for k in range(N)])
+        assert np.array_equal(m.run(am, am), want), "all q-1"
+        for s, kind in enumerate(["uniform", "mixed", "sparse"]):
+            a, b = O.operands(kind, N, W, q, 6300 + s + W)
+            want = P.multiply(a, b)
+            got = m.run(a, b)
+            assert np.array_equal(got, want), "host path, %s: %d of %d differ" % (kind, _diff(got, want), N)
+            da, db = pm.DeviceBuffer.from_array(a), pm.DeviceBuffer.from_array(b)
+            dc = pm.DeviceBuffer(N * L * 4)
+            m.run_device(da, db, dc)
+            m.sync()
+            assert np.array_equal(dc.download((N, L)), want), "device path, " + kind
+
+
 def test_multiply_schoolbook_small(O, pm):
     """Independent of the oracle's NTT: schoolbook negacyclic products."""
     for N, W in [(1, 8), (8, 33), (64, 32), (128, 256)]:
