@@ -156,3 +156,7 @@ def test_batch_refuses_wide_and_long(O, pm):
         pm.Batch([(256, 4065, q)])
     with pytest.raises(pm.PolymultError, match="N above 65536"):
         pm.Batch([(131072, 64, O.spec_q(131072, 64))])
+    with pytest.raises(pm.PolymultError, match="W above 4064"):
+        pm.Server([(4096, 256, O.spec_q(4096, 256)), (256, 4065, q)])
+    with pytest.raises(pm.PolymultError, match="N above 65536"):
+        pm.Server([(131072, 64, O.spec_q(131072, 64))])
