@@ -5,9 +5,11 @@
 //   * a is cut into growing contiguous coefficient chunks (1, 3, then 16 MiB)
 //     and b goes as one copy: a host worker team stages a chunk from the
 //     caller's pageable buffer into pinned memory (non-temporal stores) at
-//     about 4x the link rate while the copy engine moves the one before and
-//     the RNS kernel decomposes the one before that; the engines run fastest
-//     on large copies, and all of b is staged during a's last copy;
+//     about 2-4x the link rate while the copy engine moves the one before;
+//     the engines run fastest on large copies, and all of b is staged during
+//     a's last copy. The RNS launches follow once every copy is queued (each
+//     waits on the device for its own chunk), so no host-side wait sits
+//     between staging one chunk and the next;
 //   * a's forward column pass runs once a is in (while b crosses the link);
 //     once b is resident the other passes run;
 //   * the result leaves in chunks over two copy streams: CRT of the first
@@ -287,7 +289,12 @@ void State::run_host(const uint32_t* a, const uint32_t* b, uint32_t* c) {
         });
     };
 
-    // ---- input: stage, H2D and RNS per chunk ----
+    // ---- input: stage and H2D per chunk, every copy queued first ----
+    // A cross-stream hipStreamWaitEvent costs 11-20 us of host time, about a
+    // 1 MiB copy's worth of link time, and it would sit between staging one
+    // chunk and the next. So the loop queues only the copies (and a cheap
+    // event after each); the waits and the RNS launches follow once the last
+    // chunk is staged, while b is still crossing the link.
     acquire_R(stream);  // d_R: after any caller-stream or sharded call still in flight
     if (P.profile) hip_check(hipEventRecord(P.tl[0], P.s_in), "record");
     int i = -1;  // running input chunk index over both operands
@@ -299,13 +306,20 @@ void State::run_host(const uint32_t* a, const uint32_t* b, uint32_t* c) {
             hipStream_t sc = (P.s_in2 && (i & 1)) ? P.s_in2 : P.s_in;
             hip_check(hipMemcpyAsync(d_in[op] + off, P.h_in[op] + off, words * 4, hipMemcpyHostToDevice, sc), "H2D");
             hip_check(hipEventRecord(P.ev_in[i], sc), "record");
+        }
+    }
+    const auto t_in = clock::now();
+    // ---- RNS per chunk, each after its copy; a's column pass before b's RNS ----
+    i = -1;
+    for (int op = 0; op < 2; ++op) {
+        for (const auto ch : P.cin[op]) {
+            ++i;
             hip_check(hipStreamWaitEvent(stream, P.ev_in[i], 0), "wait");
             hip_check(launch_rns(dev, cfg, d_in[op], nullptr, d_R[op], nullptr, 1, stream, ch.first, ch.count), "rns");
         }
-        // a's column pass runs while b's chunks are still crossing the link
+        // a's column pass runs while b is still crossing the link
         if (op == 0) hip_check(launch_col(dev, cfg, d_R[0], nullptr, 1, false, stream), "col a");
     }
-    const auto t_in = clock::now();
     if (P.profile) {
         if (P.s_in2) hip_check(hipStreamWaitEvent(P.s_in, P.ev_in[i], 0), "wait");
         hip_check(hipEventRecord(P.tl[1], P.s_in), "record");
