@@ -119,10 +119,18 @@ struct Piece {
 };
 constexpr size_t kPiece = 2u << 20;
 
-void add_pieces(std::vector<Piece>& v, void* dst, const void* src, size_t n) {
-    for (size_t o = 0; o < n; o += kPiece)
-        v.push_back({static_cast<char*>(dst) + o, static_cast<const char*>(src) + o, std::min(kPiece, n - o)});
+void add_pieces(std::vector<Piece>& v, void* dst, const void* src, size_t n, size_t piece = kPiece) {
+    for (size_t o = 0; o < n; o += piece)
+        v.push_back({static_cast<char*>(dst) + o, static_cast<const char*>(src) + o, std::min(piece, n - o)});
 }
+
+// The host-to-device copy of a batch's first request goes in chunks of these
+// sizes (the last repeats): the copy engine starts on the first MiB while the
+// team stages the next chunk, instead of waiting for the whole batch to be
+// staged; large chunks after that keep the engine near its large-copy rate.
+// Each later request goes as one copy: its staging already overlaps the
+// copies of the requests before it.
+constexpr size_t kCopyChunks[] = {1u << 20, 3u << 20, 16u << 20};
 
 }  // namespace
 
@@ -425,16 +433,6 @@ std::vector<Request*> Server::take_locked() {
 
 void Server::launch(Slot& S) {
     unsigned char* dev = static_cast<unsigned char*>(d_arena);
-    // stage a and b of every request into pinned memory
-    std::vector<Piece> pieces;
-    for (Request* r : S.reqs) {
-        const size_t n = cat.io_words(r->point) * 4;
-        add_pieces(pieces, h_arena + r->pin_off, r->a, n);
-        add_pieces(pieces, h_arena + r->pin_off + io_bytes[r->point], r->b, n);
-    }
-    stage_team->run((int)pieces.size(), [&](int i) { copy_stream(pieces[i].dst, pieces[i].src, pieces[i].n); });
-    S.t_staged = now_s();
-
     Runner& R = S.run;
     R.prods.assign(S.reqs.size(), BatchProduct{});
     for (size_t i = 0; i < S.reqs.size(); ++i) {
@@ -454,10 +452,32 @@ void Server::launch(Slot& S) {
     // queue of later batches' operands. The slot's tile buffers are free: a
     // slot is reused only after its batch retired.
     R.upload(s_in, true);
-    for (Request* r : S.reqs)
-        hip_check(hipMemcpyAsync(dev + r->dev_off, h_arena + r->pin_off,
-                                 io_bytes[r->point] + cat.io_words(r->point) * 4, hipMemcpyHostToDevice, s_in),
-                  "H2D");
+    // Stage each request's a and b into pinned memory and copy them in, chunk
+    // by chunk: a request's a|b region (a, its padding to 256 bytes, b) is
+    // contiguous in pinned and in device memory alike.
+    std::vector<Piece> pieces;
+    const int T = stage_team->size();
+    for (Request* r : S.reqs) {
+        const size_t n = cat.io_words(r->point) * 4, ib = io_bytes[r->point], span = ib + n;
+        unsigned char* h = h_arena + r->pin_off;
+        const bool first = r == S.reqs.front();
+        size_t k = 0;
+        for (size_t lo = 0, hi; lo < span; lo = hi) {
+            hi = first ? std::min(span, lo + kCopyChunks[k]) : span;
+            if (k + 1 < sizeof(kCopyChunks) / sizeof(kCopyChunks[0])) ++k;
+            const size_t piece = std::max<size_t>(64u << 10, (hi - lo + T - 1) / T);
+            pieces.clear();
+            // the parts of a = [0, n) and b = [ib, ib + n) inside [lo, hi)
+            if (lo < n) add_pieces(pieces, h + lo, reinterpret_cast<const char*>(r->a) + lo, std::min(hi, n) - lo, piece);
+            if (hi > ib) {
+                const size_t b0 = std::max(lo, ib);
+                add_pieces(pieces, h + b0, reinterpret_cast<const char*>(r->b) + (b0 - ib), hi - b0, piece);
+            }
+            stage_team->run((int)pieces.size(), [&](int i) { copy_stream(pieces[i].dst, pieces[i].src, pieces[i].n); });
+            hip_check(hipMemcpyAsync(dev + r->dev_off + lo, h + lo, hi - lo, hipMemcpyHostToDevice, s_in), "H2D");
+        }
+    }
+    S.t_staged = now_s();
     hip_check(hipEventRecord(S.ev_in, s_in), "record");
     hipStream_t sc = s_comp[batch_seq++ % 2];
     hip_check(hipStreamWaitEvent(sc, S.ev_in, 0), "wait");
