@@ -132,6 +132,16 @@ void add_pieces(std::vector<Piece>& v, void* dst, const void* src, size_t n, siz
 // copies of the requests before it.
 constexpr size_t kCopyChunks[] = {1u << 20, 3u << 20, 16u << 20};
 
+// The results go back the same way: the first request's in chunks of these
+// sizes (the last repeats) closed by a 0.5 MiB tail, each later request in
+// one copy, with an event after each of the first request's chunks and after
+// about every MiB of later results. The completion thread scatters each marked
+// run into the callers' buffers while the copies after it are still running,
+// so only the tail's scatter follows the last copy.
+constexpr size_t kOutChunks[] = {1u << 20, 5u << 20};
+constexpr size_t kOutTail = 512u << 10;
+constexpr size_t kOutMark = 1u << 20;
+
 }  // namespace
 
 struct Server {
@@ -159,6 +169,12 @@ struct Server {
         std::vector<Request*> reqs;
         hipEvent_t ev_in = nullptr, ev_out = nullptr;
         double t_take = 0, t_staged = 0, t_issued = 0;
+        // Results leave in marked runs of copies: mark k's event follows its
+        // copies, and its scatter pieces are out[out_end[k - 1] .. out_end[k]).
+        std::vector<hipEvent_t> mark_ev;
+        std::vector<size_t> out_end;
+        std::vector<Piece> out;
+        size_t nmark = 0;
     };
     std::vector<std::unique_ptr<Slot>> slots;
 
@@ -214,7 +230,7 @@ struct Server {
     void complete_loop();
     std::vector<Request*> take_locked();
     void launch(Slot& S);
-    void retire(Slot& S, hipError_t status);
+    void retire(Slot& S, hipError_t status, double t_seen);
     void fail_batch(Slot& S, const std::string& why);
 };
 
@@ -295,9 +311,11 @@ Server::~Server() {
     if (!trace_path.empty()) write_trace();
     for (hipStream_t s : {s_in, s_comp[0], s_comp[1]})
         if (s) (void)hipStreamSynchronize(s);
-    for (auto& S : slots)
+    for (auto& S : slots) {
         for (hipEvent_t e : {S->ev_in, S->ev_out})
             if (e) (void)hipEventDestroy(e);
+        for (hipEvent_t e : S->mark_ev) (void)hipEventDestroy(e);
+    }
     slots.clear();
     for (hipStream_t s : {s_in, s_comp[0], s_comp[1]})
         if (s) (void)hipStreamDestroy(s);
@@ -482,24 +500,52 @@ void Server::launch(Slot& S) {
     hipStream_t sc = s_comp[batch_seq++ % 2];
     hip_check(hipStreamWaitEvent(sc, S.ev_in, 0), "wait");
     hip_check(R.launch(sc), "batch launch");
-    for (Request* r : S.reqs)
-        hip_check(hipMemcpyAsync(h_arena + r->pin_off, dev + r->dev_off, cat.io_words(r->point) * 4,
-                                 hipMemcpyDeviceToHost, sc),
-                  "D2H");
+    S.out.clear();
+    S.out_end.clear();
+    S.nmark = 0;
+    const int Ts = scatter_team->size();
+    size_t pending = 0;
+    auto mark = [&] {
+        if (S.nmark == S.mark_ev.size()) {
+            hipEvent_t e = nullptr;
+            hip_check(hipEventCreateWithFlags(&e, hipEventDisableTiming), "event");
+            S.mark_ev.push_back(e);
+        }
+        hip_check(hipEventRecord(S.mark_ev[S.nmark++], sc), "record");
+        S.out_end.push_back(S.out.size());
+        pending = 0;
+    };
+    for (Request* r : S.reqs) {
+        const size_t n = cat.io_words(r->point) * 4;
+        unsigned char* h = h_arena + r->pin_off;
+        const bool first = r == S.reqs.front();
+        size_t k = 0;
+        for (size_t lo = 0, hi; lo < n; lo = hi) {
+            hi = n;
+            if (first && n - lo > kOutTail) {
+                // a chunk of the plan, but never leaving less than the tail
+                hi = std::min(lo + kOutChunks[k], n - kOutTail);
+                if (k + 1 < sizeof(kOutChunks) / sizeof(kOutChunks[0])) ++k;
+            }
+            hip_check(hipMemcpyAsync(h + lo, dev + r->dev_off + lo, hi - lo, hipMemcpyDeviceToHost, sc), "D2H");
+            add_pieces(S.out, reinterpret_cast<char*>(r->c) + lo, h + lo, hi - lo,
+                       std::max<size_t>(64u << 10, (hi - lo + Ts - 1) / Ts));
+            pending += hi - lo;
+            if (first || pending >= kOutMark) mark();
+        }
+    }
+    if (pending) mark();
     hip_check(hipEventRecord(S.ev_out, sc), "record");
     S.t_issued = now_s();
 }
 
-// Called by the completer once the batch's last D2H has finished (or failed).
-void Server::retire(Slot& S, hipError_t status) {
+// Called by the completer once the batch's results are in the callers'
+// buffers (t_seen: when its last D2H was seen finished), or on a failure.
+void Server::retire(Slot& S, hipError_t status, double t_seen) {
     if (status != hipSuccess) {
         fail_batch(S, std::string("batch failed: ") + hipGetErrorString(status));
         return;
     }
-    const double t_seen = now_s();
-    std::vector<Piece> pieces;
-    for (Request* r : S.reqs) add_pieces(pieces, r->c, h_arena + r->pin_off, cat.io_words(r->point) * 4);
-    scatter_team->run((int)pieces.size(), [&](int i) { copy_stream(pieces[i].dst, pieces[i].src, pieces[i].n); });
     const double t = now_s();
     {
         std::lock_guard<std::mutex> lk(m);
@@ -622,8 +668,8 @@ void Server::schedule_loop() {
     }
 }
 
-// Retires issued batches in order: waits for the last D2H, scatters the
-// results, releases the memory and hands the slot back.
+// Retires issued batches in order: scatters each marked run of results once
+// its copies have finished, then releases the memory and hands the slot back.
 void Server::complete_loop() {
     for (;;) {
         Slot* S = nullptr;
@@ -633,9 +679,21 @@ void Server::complete_loop() {
             if (inflight.empty()) return;
             S = inflight.front();
         }
-        hipError_t q;
-        while ((q = hipEventQuery(S->ev_out)) == hipErrorNotReady) relax();
-        retire(*S, q);
+        // each marked run of copies, scattered as soon as its event completes
+        hipError_t q = hipSuccess;
+        double t_seen = 0;
+        for (size_t k = 0, p0 = 0; k < S->nmark; ++k) {
+            while ((q = hipEventQuery(S->mark_ev[k])) == hipErrorNotReady) relax();
+            if (q != hipSuccess) break;
+            if (k + 1 == S->nmark) t_seen = now_s();
+            const Piece* pc = S->out.data() + p0;
+            const size_t p1 = S->out_end[k];
+            scatter_team->run((int)(p1 - p0), [&](int i) { copy_stream(pc[i].dst, pc[i].src, pc[i].n); });
+            p0 = p1;
+        }
+        if (q == hipSuccess)
+            while ((q = hipEventQuery(S->ev_out)) == hipErrorNotReady) relax();
+        retire(*S, q, t_seen);
         {
             std::lock_guard<std::mutex> lk(m);
             inflight.pop_front();
