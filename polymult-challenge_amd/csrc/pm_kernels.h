@@ -57,7 +57,7 @@ struct LaunchCfg {
     int row_rpb = 16;   // rows per row-pass block
 };
 
-enum RowMode { ROW_FUSED = 0, ROW_FWD = 1, ROW_INV = 2 };
+enum RowMode { ROW_FUSED = 0, ROW_FWD = 1, ROW_INV = 2, ROW_FUSED_PRE = 3 };
 
 // Chooses block shapes for a plan: tiles sized to `max_lds`, the CRT split
 // when its block exceeds `device_lds` (the device's LDS per block).
@@ -73,7 +73,8 @@ hipError_t launch_rns(const DevPlan& D, const LaunchCfg& C, const uint32_t* a0, 
 hipError_t launch_col(const DevPlan& D, const LaunchCfg& C, uint32_t* R0, uint32_t* R1, int nops, bool inverse,
                       hipStream_t s);
 // Row pass. ROW_FUSED: RB <- inv(fwd(RB) * fwd(RA)) per row (RA untouched,
-// the product carries 2^-32). ROW_FWD / ROW_INV: RB only.
+// the product carries 2^-32). ROW_FUSED_PRE: the same with RA already through
+// ROW_FWD (so a's rows can run before b is in). ROW_FWD / ROW_INV: RB only.
 hipError_t launch_row(const DevPlan& D, const LaunchCfg& C, uint32_t* RB, const uint32_t* RA, RowMode mode,
                       hipStream_t s);
 // out[n][L] = CRT of R[j][n] (scaled by G) mod q, for n in [first, first + count).

@@ -367,6 +367,53 @@ __device__ __forceinline__ void row_tile(const DevPlan& D, u32* __restrict__ RB,
 #pragma unroll
             for (int k = 0; k < E; ++k) rb[t + k * T] = y[k];
         }
+    } else if constexpr (MODE == ROW_FUSED_PRE) {
+        // b's forward rows; a's row is already in the NTT domain, each
+        // position where ROW_FWD left it (t E + k after a second register
+        // pass, t + k T otherwise), which is where this thread's b value sits
+        const u32* ra = RA + off;
+        const u32 pinv = D.pinv[j];
+        u32 x[1][E];
+        if (on) {
+#pragma unroll
+            for (int k = 0; k < E; ++k) x[0][k] = rb[t + k * T];
+            fwd_pass1<NB, 1>(x, tab, s0, row, p);
+        }
+        if constexpr (R2 > 0) {
+            if (on) {
+#pragma unroll
+                for (int k = 0; k < E; ++k) lb[ps(k)] = x[0][k];
+            }
+            __syncthreads();
+            if (on) {
+#pragma unroll
+                for (int k = 0; k < E; ++k) x[0][k] = lb[pc(k)];
+                fwd_pass2<NB, 1>(x, tab, s0, row, t, p);
+            }
+        }
+        u32 y[E];
+        if (on) {
+#pragma unroll
+            for (int k = 0; k < E; ++k) y[k] = mont_mul(R2 > 0 ? ra[t * E + k] : ra[t + k * T], x[0][k], p, pinv);
+        }
+        if constexpr (R2 > 0) {
+            if (on) inv_pass2<NB>(y, tab, s0, row, t, p);
+            __syncthreads();
+            if (on) {
+#pragma unroll
+                for (int k = 0; k < E; ++k) lb[pc(k)] = y[k];
+            }
+            __syncthreads();
+            if (on) {
+#pragma unroll
+                for (int k = 0; k < E; ++k) y[k] = lb[ps(k)];
+            }
+        }
+        if (on) {
+            inv_pass1<NB>(y, tab, s0, row, p);
+#pragma unroll
+            for (int k = 0; k < E; ++k) rb[t + k * T] = y[k];
+        }
     } else if constexpr (MODE == ROW_FWD) {
         u32 x[1][E];
         if (on) {
@@ -1108,6 +1155,7 @@ hipError_t launch_row(const DevPlan& D, const LaunchCfg& C, u32* RB, const u32* 
         case ROW_FUSED: return row_dispatch<ROW_FUSED>(D.w2, grid, block, lds, s, D, RB, RA, C.row_rpb);
         case ROW_FWD: return row_dispatch<ROW_FWD>(D.w2, grid, block, lds, s, D, RB, RA, C.row_rpb);
         case ROW_INV: return row_dispatch<ROW_INV>(D.w2, grid, block, lds, s, D, RB, RA, C.row_rpb);
+        case ROW_FUSED_PRE: return row_dispatch<ROW_FUSED_PRE>(D.w2, grid, block, lds, s, D, RB, RA, C.row_rpb);
     }
     return hipErrorInvalidValue;
 }

@@ -10,8 +10,9 @@
 //     a's last copy. The RNS launches follow once every copy is queued (each
 //     waits on the device for its own chunk), so no host-side wait sits
 //     between staging one chunk and the next;
-//   * a's forward column pass runs once a is in (while b crosses the link);
-//     once b is resident the other passes run;
+//   * a's forward column pass and forward rows run once a is in (while b
+//     crosses the link); once b is resident the other passes run, the row
+//     pass transforming only b's rows before the pointwise product;
 //   * the result leaves in chunks over two copy streams: CRT of the first
 //     chunk alone, then the rest, device-to-host copies alternating between
 //     the streams, and the host team scatters each chunk from pinned staging
@@ -317,8 +318,11 @@ void State::run_host(const uint32_t* a, const uint32_t* b, uint32_t* c) {
             hip_check(hipStreamWaitEvent(stream, P.ev_in[i], 0), "wait");
             hip_check(launch_rns(dev, cfg, d_in[op], nullptr, d_R[op], nullptr, 1, stream, ch.first, ch.count), "rns");
         }
-        // a's column pass runs while b is still crossing the link
-        if (op == 0) hip_check(launch_col(dev, cfg, d_R[0], nullptr, 1, false, stream), "col a");
+        // a's column pass and forward rows run while b is still crossing the link
+        if (op == 0) {
+            hip_check(launch_col(dev, cfg, d_R[0], nullptr, 1, false, stream), "col a");
+            hip_check(launch_row(dev, cfg, d_R[0], nullptr, ROW_FWD, stream), "rows a");
+        }
     }
     if (P.profile) {
         if (P.s_in2) hip_check(hipStreamWaitEvent(P.s_in, P.ev_in[i], 0), "wait");
@@ -328,7 +332,7 @@ void State::run_host(const uint32_t* a, const uint32_t* b, uint32_t* c) {
 
     // ---- the rest of the NTT passes ----
     hip_check(launch_col(dev, cfg, d_R[1], nullptr, 1, false, stream), "col b");
-    hip_check(launch_row(dev, cfg, d_R[1], d_R[0], ROW_FUSED, stream), "row");
+    hip_check(launch_row(dev, cfg, d_R[1], d_R[0], ROW_FUSED_PRE, stream), "row");
     hip_check(launch_col(dev, cfg, d_R[1], nullptr, 1, true, stream), "col inv");
     if (P.profile) hip_check(hipEventRecord(P.tl_more[0], stream), "record");
 
