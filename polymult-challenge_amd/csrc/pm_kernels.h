@@ -53,6 +53,7 @@ struct LaunchCfg {
     // not fit the LDS either (combine_direct).
     bool crt_split = false;
     bool combine_direct = false;
+    bool crt_alias = false;  // k_crt_alias: slots over the y bytes (32 coefficients, crt_rt <= 8)
     int col_tc = 32;    // columns per column-pass block
     int row_rpb = 16;   // rows per row-pass block
 };
@@ -147,6 +148,13 @@ inline size_t crt_combine_lds_bytes(int tn, int L, bool direct) {
     return (size_t)tn * ((direct ? 0 : (size_t)(L + 3) * 8) + (size_t)(L + 2) * 4);
 }
 constexpr int kBatchCrtTile = 32;
+// k_crt_alias (32 coefficients): the y bytes, and over them the slot rows
+// [32][2 (L + 3) + 1] words, then the constants at a 16-byte boundary.
+inline size_t crt_alias_kc_off(int ys, int L) {
+    const size_t y = (size_t)32 * ys, slots = (size_t)32 * (2 * (L + 3) + 1) * 4;
+    return ((y > slots ? y : slots) + 15) / 16 * 16;
+}
+inline size_t crt_alias_lds_bytes(int ys, int L, int crt_ks) { return crt_alias_kc_off(ys, L) + (size_t)128 * crt_ks; }
 
 // Zeroes n words (the persistent tile counters) with a kernel, so the launch
 // can sit in a captured graph.

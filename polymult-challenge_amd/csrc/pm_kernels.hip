@@ -795,6 +795,131 @@ __device__ __forceinline__ void crt_tile(const DevPlan& D, const u32* __restrict
                reinterpret_cast<u32*>(Y) /* reuses the y bytes */, tid, nthreads);
 }
 
+// The finish of one coefficient, in place (the steps of crt_finish): z is
+// the coefficient's row of words, slot i (i < L + 3) at z[2 i] (low) and
+// z[2 i + 1] (high); limb i (i < L + 2) is written over slot i's low word
+// once slot i has been read.
+__device__ __forceinline__ void crt_finish_inplace(const DevPlan& D, u32* __restrict__ z) {
+    const int L = D.L, K = D.K, ZL = L + 2;
+    const int W = D.W, wl = W >> 5, wb = W & 31;
+    const u32* C0 = D.C0;
+    auto slot = [&](int i) { return (long long)(((u64)z[2 * i + 1] << 32) | z[2 * i]); };
+    const u64 F = (u64)slot(L + 1) + ((u64)slot(L + 2) << 32) + (u64)D.F0;
+    // v <= K for every valid input; the clamp only keeps a corrupted sum
+    // from indexing past T.
+    const u32 v = min((u32)((F + (1ull << 47)) >> 48), (u32)K);
+    const u32* Tv = D.T + (size_t)v * L;
+    long long carry = 0;
+#pragma unroll 8
+    for (int i = 0; i < L; ++i) {
+        const long long s = carry + slot(i) + (long long)C0[i] + (long long)Tv[i];
+        z[2 * i] = (u32)s;
+        carry = s >> 32;  // arithmetic: the running sum may be negative
+    }
+    {
+        long long s = carry + slot(L) + (long long)C0[L];
+        z[2 * L] = (u32)s;
+        s = (s >> 32) + (long long)C0[L + 1];
+        z[2 * (L + 1)] = (u32)s;
+    }
+    for (;;) {
+        u64 hi;
+        if (wb == 0) {
+            hi = (u64)z[2 * wl] | ((u64)z[2 * (wl + 1)] << 32);
+        } else {
+            hi = ((u64)z[2 * wl] >> wb) | ((u64)z[2 * (wl + 1)] << (32 - wb));
+            if (wl + 2 < ZL) hi |= (u64)z[2 * (wl + 2)] << (64 - wb);
+        }
+        if (hi == 0) break;
+        z[2 * wl] = wb ? (z[2 * wl] & ((1u << wb) - 1u)) : 0u;
+        for (int i = wl + 1; i < ZL; ++i) z[2 * i] = 0;
+        const u64 lo64 = hi * D.delta, hi64 = __umul64hi(hi, D.delta);
+        const u32 add[4] = {(u32)lo64, (u32)(lo64 >> 32), (u32)hi64, (u32)(hi64 >> 32)};
+        u64 cy = 0;
+        for (int i = 0; i < ZL; ++i) {
+            const u64 s = (u64)z[2 * i] + (i < 4 ? add[i] : 0u) + cy;
+            z[2 * i] = (u32)s;
+            cy = s >> 32;
+            if (i >= 3 && cy == 0) break;
+        }
+    }
+    bool ge = true;
+    for (int i = L - 1; i >= 0; --i) {
+        const u32 qi = D.q[i], zi = z[2 * i];
+        if (zi != qi) {
+            ge = zi > qi;
+            break;
+        }
+    }
+    if (ge) {
+        long long br = 0;
+        for (int i = 0; i < L; ++i) {
+            const long long d = (long long)z[2 * i] - D.q[i] - br;
+            z[2 * i] = (u32)d;
+            br = d < 0;
+        }
+    }
+}
+
+// The CRT with the slots over the y bytes: 32 coefficients per block of 4
+// waves, at most two GEMM row tiles per wave (crt_rt <= 8). Each wave keeps
+// its row tiles' results in registers until every wave has read the y
+// bytes, then writes them as slot rows [32][S] (S = 2 (L + 3) + 1 words, an
+// odd stride, so lanes on consecutive coefficients hit distinct banks); the
+// finish writes each limb over its slot, and the store reads the limbs in
+// place. The block's LDS is max(y bytes, slot rows) + constants instead of
+// their sum, so more blocks share a CU to hide the residue loads.
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(6, 8))) k_crt_alias(DevPlan D, const u32* __restrict__ R, u32* __restrict__ out,
+                                                    const uint2* __restrict__ G, u32 count, int ys, int kc_off) {
+    extern __shared__ __align__(16) unsigned char lds8[];
+    constexpr int TN = 32;
+    const int tid = (int)threadIdx.x, nthreads = (int)blockDim.x;
+    const u32 n0 = blockIdx.x * (u32)TN;
+    const int cnt = (int)min((u32)TN, count - n0);
+    const int L = D.L, nslot = L + 3, S = 2 * nslot + 1;
+    unsigned char* Y = lds8;                                  // [TN][ys], then the slot rows
+    uint4* kc = reinterpret_cast<uint4*>(lds8 + kc_off);      // [8 crt_ks]
+    crt_consts(D, G, kc, tid, nthreads);
+    __syncthreads();
+    crt_y<TN>(D, R, kc, n0, cnt, ys, Y, tid, nthreads);
+    __syncthreads();
+    const int lane = tid & 63, wave = tid >> 6, nw = nthreads >> 6;
+    const int half = lane >> 5, col = lane & 31;
+    auto bfrag = [&](int ct, int ks) {
+        return *reinterpret_cast<const v4i*>(Y + (32 * ct + col) * ys + 32 * ks + 16 * half);
+    };
+    v16i acc[2][1];
+#pragma unroll
+    for (int h = 0; h < 2; ++h) {
+        acc[h][0] = (v16i){0};
+        const int rt = wave + h * nw;
+        if (rt < D.crt_rt) gemm_rows<1>(acc[h], D.crt_frag + (size_t)rt * D.crt_ks_full * 64 + lane, D.crt_ks, bfrag);
+    }
+    __syncthreads();  // every wave has read the y bytes
+    u32* sl = reinterpret_cast<u32*>(lds8);
+#pragma unroll
+    for (int h = 0; h < 2; ++h) {
+        const int rt = wave + h * nw;
+        if (rt >= D.crt_rt) continue;
+#pragma unroll
+        for (int g = 0; g < 4; ++g) {
+            const int slot = 8 * rt + 2 * g + half;
+            if (slot >= nslot) continue;
+            const long long v = slot_value(acc[h][0], g);
+            sl[col * S + 2 * slot] = (u32)v;
+            sl[col * S + 2 * slot + 1] = (u32)((u64)v >> 32);
+        }
+    }
+    __syncthreads();
+    for (int c = tid; c < cnt; c += nthreads) crt_finish_inplace(D, sl + c * S);
+    __syncthreads();
+    u32* dst = out + (size_t)n0 * L;
+    for (int idx = tid; idx < cnt * L; idx += nthreads) {
+        const int c = L == 1 ? idx : (int)__umulhi((u32)idx, D.magic_L);  // idx / L
+        dst[idx] = sl[c * S + 2 * (idx - c * L)];
+    }
+}
+
 // One prime shard's share of every slot: P[slot][n] (int64), for the
 // coefficients [32 NCT blockIdx.x, ...) below count. D is the shard's view.
 template <int NCT>
@@ -1089,6 +1214,11 @@ LaunchCfg choose_launch(const DevPlan& D, size_t max_lds, size_t device_lds) {
         C.crt_lds = crt_partial_lds_bytes(32, C.crt_ys, D.crt_ks);
     }
     C.combine_direct = crt_combine_lds_bytes(C.crt_tn, D.L, false) > device_lds;
+    // 32-coefficient blocks with at most two row tiles per wave put the slots
+    // over the y bytes (k_crt_alias); PM_CRT_ALIAS=0 keeps the separate slots.
+    const char* ca = std::getenv("PM_CRT_ALIAS");
+    C.crt_alias = !C.crt_split && C.crt_tn == 32 && D.crt_rt <= 8 && !(ca && ca[0] == '0');
+    if (C.crt_alias) C.crt_lds = crt_alias_lds_bytes(C.crt_ys, D.L, D.crt_ks);
     C.col_tc = (int)std::min<u32>(ncols, 32);
     C.row_rpb = (int)std::min<u32>(nrows, (u32)(256 / line_threads(D.w2)));
     return C;
@@ -1175,6 +1305,11 @@ hipError_t launch_crt(const DevPlan& D, const LaunchCfg& C, const u32* R, u32* o
     }
     R += first;
     out += (size_t)first * D.L;
+    if (C.crt_alias) {
+        hipLaunchKernelGGL(k_crt_alias, grid, block, C.crt_lds, s, D, R, out, G, count, C.crt_ys,
+                           (int)crt_alias_kc_off(C.crt_ys, D.L));
+        return hipGetLastError();
+    }
     switch (C.crt_tn) {
         case 128: hipLaunchKernelGGL((k_crt<4>), grid, block, C.crt_lds, s, D, R, out, G, count, C.crt_ys); break;
         case 64: hipLaunchKernelGGL((k_crt<2>), grid, block, C.crt_lds, s, D, R, out, G, count, C.crt_ys); break;
