@@ -483,9 +483,8 @@ k_row(DevPlan D, u32* __restrict__ RB, const u32* __restrict__ RA, int rpb) {
 // + 0..15; B lane l holds column l%32 and the same k; result register r of
 // lane l is row 8(r/4) + 4(l/32) + r%4, column l%32.
 // ---------------------------------------------------------------------------
-template <int NCT, class BF>
+template <int NCT, class BF, int PF = 8>
 __device__ __forceinline__ void gemm_rows(v16i (&acc)[NCT], const v4i* __restrict__ frag, int KS, BF bfrag) {
-    constexpr int PF = 8;
     for (int ks0 = 0; ks0 < KS; ks0 += PF) {
         v4i Af[PF];
 #pragma unroll
@@ -868,8 +867,13 @@ __device__ __forceinline__ void crt_finish_inplace(const DevPlan& D, u32* __rest
 // odd stride, so lanes on consecutive coefficients hit distinct banks); the
 // finish writes each limb over its slot, and the store reads the limbs in
 // place. The block's LDS is max(y bytes, slot rows) + constants instead of
-// their sum, so more blocks share a CU to hide the residue loads.
-__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(6, 8))) k_crt_alias(DevPlan D, const u32* __restrict__ R, u32* __restrict__ out,
+// their sum (17.8 KB instead of 32.6 KB at L = 55), so more blocks share a CU
+// to hide the residue and fragment loads.
+// Four fragments in flight per wave (instead of eight) keep the kernel at 64
+// VGPRs, so eight 4-wave blocks share a CU (the LDS allows eight): measured
+// 22.8-23.3 us against 25.5-25.7 us with eight in flight at six blocks per CU.
+constexpr int kCrtAliasPF = 4;
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(8, 8))) k_crt_alias(DevPlan D, const u32* __restrict__ R, u32* __restrict__ out,
                                                     const uint2* __restrict__ G, u32 count, int ys, int kc_off) {
     extern __shared__ __align__(16) unsigned char lds8[];
     constexpr int TN = 32;
@@ -888,12 +892,17 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(6, 8))
     auto bfrag = [&](int ct, int ks) {
         return *reinterpret_cast<const v4i*>(Y + (32 * ct + col) * ys + 32 * ks + 16 * half);
     };
-    v16i acc[2][1];
+    // each row tile's result, reduced to its lane's four slot values
+    long long sv[2][4];
 #pragma unroll
     for (int h = 0; h < 2; ++h) {
-        acc[h][0] = (v16i){0};
+        v16i acc[1] = {(v16i){0}};
         const int rt = wave + h * nw;
-        if (rt < D.crt_rt) gemm_rows<1>(acc[h], D.crt_frag + (size_t)rt * D.crt_ks_full * 64 + lane, D.crt_ks, bfrag);
+        if (rt < D.crt_rt)
+            gemm_rows<1, decltype(bfrag), kCrtAliasPF>(acc, D.crt_frag + (size_t)rt * D.crt_ks_full * 64 + lane,
+                                                        D.crt_ks, bfrag);
+#pragma unroll
+        for (int g = 0; g < 4; ++g) sv[h][g] = slot_value(acc[0], g);
     }
     __syncthreads();  // every wave has read the y bytes
     u32* sl = reinterpret_cast<u32*>(lds8);
@@ -905,9 +914,8 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(6, 8))
         for (int g = 0; g < 4; ++g) {
             const int slot = 8 * rt + 2 * g + half;
             if (slot >= nslot) continue;
-            const long long v = slot_value(acc[h][0], g);
-            sl[col * S + 2 * slot] = (u32)v;
-            sl[col * S + 2 * slot + 1] = (u32)((u64)v >> 32);
+            sl[col * S + 2 * slot] = (u32)sv[h][g];
+            sl[col * S + 2 * slot + 1] = (u32)((u64)sv[h][g] >> 32);
         }
     }
     __syncthreads();
