@@ -148,13 +148,15 @@ inline size_t crt_combine_lds_bytes(int tn, int L, bool direct) {
     return (size_t)tn * ((direct ? 0 : (size_t)(L + 3) * 8) + (size_t)(L + 2) * 4);
 }
 constexpr int kBatchCrtTile = 32;
-// k_crt_alias (32 coefficients): the y bytes, and over them the slot rows
-// [32][2 (L + 3) + 1] words, then the constants at a 16-byte boundary.
-inline size_t crt_alias_kc_off(int ys, int L) {
-    const size_t y = (size_t)32 * ys, slots = (size_t)32 * (2 * (L + 3) + 1) * 4;
+// k_crt_alias (tn coefficients): the y bytes, and over them the slot rows
+// [tn][2 (L + 3) + 1] words, then the constants at a 16-byte boundary.
+inline size_t crt_alias_kc_off(int tn, int ys, int L) {
+    const size_t y = (size_t)tn * ys, slots = (size_t)tn * (2 * (L + 3) + 1) * 4;
     return ((y > slots ? y : slots) + 15) / 16 * 16;
 }
-inline size_t crt_alias_lds_bytes(int ys, int L, int crt_ks) { return crt_alias_kc_off(ys, L) + (size_t)128 * crt_ks; }
+inline size_t crt_alias_lds_bytes(int tn, int ys, int L, int crt_ks) {
+    return crt_alias_kc_off(tn, ys, L) + (size_t)128 * crt_ks;
+}
 
 // Zeroes n words (the persistent tile counters) with a kernel, so the launch
 // can sit in a captured graph.

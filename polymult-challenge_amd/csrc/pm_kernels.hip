@@ -860,23 +860,28 @@ __device__ __forceinline__ void crt_finish_inplace(const DevPlan& D, u32* __rest
     }
 }
 
-// The CRT with the slots over the y bytes: 32 coefficients per block of 4
-// waves, at most two GEMM row tiles per wave (crt_rt <= 8). Each wave keeps
-// its row tiles' results in registers until every wave has read the y
-// bytes, then writes them as slot rows [32][S] (S = 2 (L + 3) + 1 words, an
-// odd stride, so lanes on consecutive coefficients hit distinct banks); the
-// finish writes each limb over its slot, and the store reads the limbs in
-// place. The block's LDS is max(y bytes, slot rows) + constants instead of
-// their sum (17.8 KB instead of 32.6 KB at L = 55), so more blocks share a CU
-// to hide the residue and fragment loads.
-// Four fragments in flight per wave (instead of eight) keep the kernel at 64
-// VGPRs, so eight 4-wave blocks share a CU (the LDS allows eight): measured
-// 22.8-23.3 us against 25.5-25.7 us with eight in flight at six blocks per CU.
+// The CRT with the slots over the y bytes: TN = 32 NCT coefficients per
+// block of 4 NCT waves, so the eight GEMM row tiles (crt_rt <= 8) make two
+// per wave at NCT = 1 and one per wave over two column tiles at NCT = 2.
+// Each wave keeps its results (reduced to slot values) in registers until
+// every wave has read the y bytes, then writes them as slot rows [TN][S]
+// (S = 2 (L + 3) + 1 words, an odd stride, so lanes on consecutive
+// coefficients hit distinct banks); the finish writes each limb over its
+// slot, and the store reads the limbs in place. The block's LDS is max(y
+// bytes, slot rows) + constants instead of their sum (17.8 KB instead of
+// 32.6 KB at L = 55, NCT = 1), so more blocks share a CU to hide the residue
+// and fragment loads; at NCT = 2 each digit fragment feeds two MFMAs, which
+// halves the fragment traffic from L2 per coefficient.
+// Four fragments in flight per wave keep NCT = 1 at 64 VGPRs: eight 4-wave
+// blocks per CU, 22.8-23.3 us against 25.5-25.7 us with eight in flight at
+// six blocks per CU (65536/1747).
 constexpr int kCrtAliasPF = 4;
-__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(8, 8))) k_crt_alias(DevPlan D, const u32* __restrict__ R, u32* __restrict__ out,
-                                                    const uint2* __restrict__ G, u32 count, int ys, int kc_off) {
+template <int NCT>
+__global__ void __launch_bounds__(256 * NCT) __attribute__((amdgpu_waves_per_eu(8, 8)))
+k_crt_alias(DevPlan D, const u32* __restrict__ R, u32* __restrict__ out, const uint2* __restrict__ G, u32 count, int ys,
+            int kc_off) {
     extern __shared__ __align__(16) unsigned char lds8[];
-    constexpr int TN = 32;
+    constexpr int TN = 32 * NCT, RTW = 2 / NCT;  // coefficients per block, row tiles per wave
     const int tid = (int)threadIdx.x, nthreads = (int)blockDim.x;
     const u32 n0 = blockIdx.x * (u32)TN;
     const int cnt = (int)min((u32)TN, count - n0);
@@ -892,31 +897,38 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(8, 8))
     auto bfrag = [&](int ct, int ks) {
         return *reinterpret_cast<const v4i*>(Y + (32 * ct + col) * ys + 32 * ks + 16 * half);
     };
-    // each row tile's result, reduced to its lane's four slot values
-    long long sv[2][4];
+    // each row tile's result, reduced to its lane's four slot values per column tile
+    long long sv[RTW][NCT][4];
 #pragma unroll
-    for (int h = 0; h < 2; ++h) {
-        v16i acc[1] = {(v16i){0}};
+    for (int h = 0; h < RTW; ++h) {
+        v16i acc[NCT];
+#pragma unroll
+        for (int ct = 0; ct < NCT; ++ct) acc[ct] = (v16i){0};
         const int rt = wave + h * nw;
         if (rt < D.crt_rt)
-            gemm_rows<1, decltype(bfrag), kCrtAliasPF>(acc, D.crt_frag + (size_t)rt * D.crt_ks_full * 64 + lane,
-                                                        D.crt_ks, bfrag);
+            gemm_rows<NCT, decltype(bfrag), kCrtAliasPF>(acc, D.crt_frag + (size_t)rt * D.crt_ks_full * 64 + lane,
+                                                          D.crt_ks, bfrag);
 #pragma unroll
-        for (int g = 0; g < 4; ++g) sv[h][g] = slot_value(acc[0], g);
+        for (int ct = 0; ct < NCT; ++ct)
+#pragma unroll
+            for (int g = 0; g < 4; ++g) sv[h][ct][g] = slot_value(acc[ct], g);
     }
     __syncthreads();  // every wave has read the y bytes
     u32* sl = reinterpret_cast<u32*>(lds8);
 #pragma unroll
-    for (int h = 0; h < 2; ++h) {
+    for (int h = 0; h < RTW; ++h) {
         const int rt = wave + h * nw;
         if (rt >= D.crt_rt) continue;
 #pragma unroll
-        for (int g = 0; g < 4; ++g) {
-            const int slot = 8 * rt + 2 * g + half;
-            if (slot >= nslot) continue;
-            sl[col * S + 2 * slot] = (u32)sv[h][g];
-            sl[col * S + 2 * slot + 1] = (u32)((u64)sv[h][g] >> 32);
-        }
+        for (int ct = 0; ct < NCT; ++ct)
+#pragma unroll
+            for (int g = 0; g < 4; ++g) {
+                const int slot = 8 * rt + 2 * g + half;
+                if (slot >= nslot) continue;
+                const int c = 32 * ct + col;
+                sl[c * S + 2 * slot] = (u32)sv[h][ct][g];
+                sl[c * S + 2 * slot + 1] = (u32)((u64)sv[h][ct][g] >> 32);
+            }
     }
     __syncthreads();
     for (int c = tid; c < cnt; c += nthreads) crt_finish_inplace(D, sl + c * S);
@@ -1224,9 +1236,15 @@ LaunchCfg choose_launch(const DevPlan& D, size_t max_lds, size_t device_lds) {
     C.combine_direct = crt_combine_lds_bytes(C.crt_tn, D.L, false) > device_lds;
     // 32-coefficient blocks with at most two row tiles per wave put the slots
     // over the y bytes (k_crt_alias); PM_CRT_ALIAS=0 keeps the separate slots.
+    // 64 coefficients per block of 8 waves where that fits the LDS budget
+    // (measured 20.5-20.7 us against 22.7-22.9 us for 32 at 65536/1747);
+    // PM_CRT_ALIAS=32 keeps 32, PM_CRT_ALIAS=0 the separate slots.
     const char* ca = std::getenv("PM_CRT_ALIAS");
     C.crt_alias = !C.crt_split && C.crt_tn == 32 && D.crt_rt <= 8 && !(ca && ca[0] == '0');
-    if (C.crt_alias) C.crt_lds = crt_alias_lds_bytes(C.crt_ys, D.L, D.crt_ks);
+    if (C.crt_alias) {
+        if (!(ca && std::atoi(ca) == 32) && crt_alias_lds_bytes(64, C.crt_ys, D.L, D.crt_ks) <= max_lds) C.crt_tn = 64;
+        C.crt_lds = crt_alias_lds_bytes(C.crt_tn, C.crt_ys, D.L, D.crt_ks);
+    }
     C.col_tc = (int)std::min<u32>(ncols, 32);
     C.row_rpb = (int)std::min<u32>(nrows, (u32)(256 / line_threads(D.w2)));
     return C;
@@ -1314,8 +1332,11 @@ hipError_t launch_crt(const DevPlan& D, const LaunchCfg& C, const u32* R, u32* o
     R += first;
     out += (size_t)first * D.L;
     if (C.crt_alias) {
-        hipLaunchKernelGGL(k_crt_alias, grid, block, C.crt_lds, s, D, R, out, G, count, C.crt_ys,
-                           (int)crt_alias_kc_off(C.crt_ys, D.L));
+        const int koff = (int)crt_alias_kc_off(C.crt_tn, C.crt_ys, D.L);
+        if (C.crt_tn == 64)
+            hipLaunchKernelGGL(k_crt_alias<2>, grid, dim3(512), C.crt_lds, s, D, R, out, G, count, C.crt_ys, koff);
+        else
+            hipLaunchKernelGGL(k_crt_alias<1>, grid, block, C.crt_lds, s, D, R, out, G, count, C.crt_ys, koff);
         return hipGetLastError();
     }
     switch (C.crt_tn) {
