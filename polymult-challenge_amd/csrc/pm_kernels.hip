@@ -876,12 +876,12 @@ __device__ __forceinline__ void crt_finish_inplace(const DevPlan& D, u32* __rest
 // blocks per CU, 22.8-23.3 us against 25.5-25.7 us with eight in flight at
 // six blocks per CU (65536/1747).
 constexpr int kCrtAliasPF = 4;
-template <int NCT>
-__global__ void __launch_bounds__(256 * NCT) __attribute__((amdgpu_waves_per_eu(8, 8)))
+template <int NCT, int MINW>  // NCT 1 or 2; MINW: waves per SIMD the registers must allow
+__global__ void __launch_bounds__(NCT == 1 ? 256 : 512) __attribute__((amdgpu_waves_per_eu(MINW, 8)))
 k_crt_alias(DevPlan D, const u32* __restrict__ R, u32* __restrict__ out, const uint2* __restrict__ G, u32 count, int ys,
             int kc_off) {
     extern __shared__ __align__(16) unsigned char lds8[];
-    constexpr int TN = 32 * NCT, RTW = 2 / NCT;  // coefficients per block, row tiles per wave
+    constexpr int TN = 32 * NCT, RTW = NCT == 1 ? 2 : 1;  // coefficients per block, row tiles per wave
     const int tid = (int)threadIdx.x, nthreads = (int)blockDim.x;
     const u32 n0 = blockIdx.x * (u32)TN;
     const int cnt = (int)min((u32)TN, count - n0);
@@ -1237,8 +1237,9 @@ LaunchCfg choose_launch(const DevPlan& D, size_t max_lds, size_t device_lds) {
     // 32-coefficient blocks with at most two row tiles per wave put the slots
     // over the y bytes (k_crt_alias); PM_CRT_ALIAS=0 keeps the separate slots.
     // 64 coefficients per block of 8 waves where that fits the LDS budget
-    // (measured 20.5-20.7 us against 22.7-22.9 us for 32 at 65536/1747);
-    // PM_CRT_ALIAS=32 keeps 32, PM_CRT_ALIAS=0 the separate slots.
+    // (measured 20.5-20.7 us against 22.7-22.9 us for 32 at 65536/1747; 128
+    // per block, 65 KB of LDS, ran 20.1-20.6 us but cost 2% of three-lane
+    // throughput). PM_CRT_ALIAS=32 keeps 32, PM_CRT_ALIAS=0 the separate slots.
     const char* ca = std::getenv("PM_CRT_ALIAS");
     C.crt_alias = !C.crt_split && C.crt_tn == 32 && D.crt_rt <= 8 && !(ca && ca[0] == '0');
     if (C.crt_alias) {
@@ -1334,9 +1335,9 @@ hipError_t launch_crt(const DevPlan& D, const LaunchCfg& C, const u32* R, u32* o
     if (C.crt_alias) {
         const int koff = (int)crt_alias_kc_off(C.crt_tn, C.crt_ys, D.L);
         if (C.crt_tn == 64)
-            hipLaunchKernelGGL(k_crt_alias<2>, grid, dim3(512), C.crt_lds, s, D, R, out, G, count, C.crt_ys, koff);
+            hipLaunchKernelGGL((k_crt_alias<2, 8>), grid, dim3(512), C.crt_lds, s, D, R, out, G, count, C.crt_ys, koff);
         else
-            hipLaunchKernelGGL(k_crt_alias<1>, grid, block, C.crt_lds, s, D, R, out, G, count, C.crt_ys, koff);
+            hipLaunchKernelGGL((k_crt_alias<1, 8>), grid, block, C.crt_lds, s, D, R, out, G, count, C.crt_ys, koff);
         return hipGetLastError();
     }
     switch (C.crt_tn) {
