@@ -9,11 +9,12 @@ timed steps. The PCIe-inclusive rate of the host-buffer call (pm_run, the
 fherma_run replacement) is reported beside it, never as `value`.
 
     python bench.py [--gpus 1] [--steps 2000] [--warmup 200] [--no-cpu] [--serve-seconds 30] [--no-serve]
-                    [--legs plain,points,shard,e2e,batch,serve,cpu] [--copy-probe BYTES]
+                    [--legs plain,points,shard,e2e,batch,serve,cpu] [--copy-probe BYTES] [--settle-ms 1000]
 
-Before the timed steps the multiply runs for at least SETTLE_S (1 s) and
+Before the timed steps the multiply runs for at least --settle-ms (1 s) and
 --warmup steps, so the clocks have settled whatever --warmup is; the timed
-steps are exactly --steps.
+steps are exactly --steps. Counter runs pass --settle-ms 0 to keep the
+profiler's per-dispatch output small.
 
 Prints one JSON line.
 """
@@ -279,6 +280,7 @@ def serving(pmlib, O, rates, seconds: float, per_point: dict) -> dict:
         S.drive(200.0, 0.5, SERVE_MIX, seed=SEED)  # warm-up
         for r in rates:
             out["rates"].append(row_of(S.drive(float(r), seconds, SERVE_MIX, seed=SEED + int(r))))
+            print("bench: serve at %d req/s done" % r, file=sys.stderr, flush=True)
         # capacity: closed loop with 64 requests in flight; the achieved rate
         # is the server's throughput on this mix
         cap = row_of(S.saturate(64, min(seconds, 3.0), SERVE_MIX, seed=SEED + 1))
@@ -307,6 +309,8 @@ def main():
     ap.add_argument("--legs", default="plain,points,shard,e2e,batch,serve,cpu",
                     help="comma list of parts to run (plain is always run: it gives value)")
     ap.add_argument("--no-autotune", action="store_true", help="batched legs on the default configuration")
+    ap.add_argument("--settle-ms", type=float, default=1e3 * SETTLE_S,
+                    help="plain leg: run at least this long before the clock (default 1000)")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="BYTES",
                     help="only run the plain device copy kernel over BYTES (the counter calibration) and exit")
     args = ap.parse_args()
@@ -315,6 +319,12 @@ def main():
               "prime across GPUs, see the design notes); --gpus %d is not supported" % args.gpus, file=sys.stderr)
         sys.exit(2)
     legs = set(args.legs.split(","))
+    t_start = time.perf_counter()
+
+    def note(what):
+        # one progress line per finished part, on stderr (stdout carries the JSON line)
+        print("bench: %s done at %.1f s" % (what, time.perf_counter() - t_start), file=sys.stderr, flush=True)
+
     if args.no_serve:
         legs.discard("serve")
     if args.no_cpu:
@@ -341,10 +351,10 @@ def main():
     c_dev = dc.download((N, L))
     correct = bool(np.array_equal(c_dev, O.OraclePlan(N, W, q).multiply(a, b)))
 
-    # settle: at least SETTLE_S seconds and --warmup steps before the clock
+    # settle: at least --settle-ms and --warmup steps before the clock
     t_settle = time.perf_counter()
     done = 0
-    while done < args.warmup or time.perf_counter() - t_settle < SETTLE_S:
+    while done < args.warmup or time.perf_counter() - t_settle < 1e-3 * args.settle_ms:
         for _ in range(50):
             m.run_device(da, db, dc)
         m.sync()
@@ -390,6 +400,7 @@ def main():
                       "modmul_frac_of_valu_peak": round(mm[k] / (kms[k] * 1e-3) / PEAK_MODMUL_PER_S, 3),
                       "modmul_frac_of_measured_valu_peak": round(mm[k] / (kms[k] * 1e-3) / peak_bfly, 3)}
                   for k in KERNELS}
+    note("plain (%.1f mults/s, bit-exact %s)" % (value, correct))
 
     # the other plain points: C1 (the CPU-reference comparison point) and C2
     # (four operand sets)
@@ -399,6 +410,7 @@ def main():
                   "C2": point_line(pmlib, O, 32768, 868, ["uniform", "max", "sparse", "small16"], SEED + 21)}
         if "cpu" in legs:
             points["C1"]["cpu_baseline"] = cpu_point(O, 4096, 256, 100, SEED + 31)
+        note("points")
 
     # prime shards: what one GPU of an 8-GPU prime-sharded multiply runs
     shard = None
@@ -417,6 +429,7 @@ def main():
                  "bit_exact_vs_plain": bool(np.array_equal(dc.download((N, L)), c_dev)),
                  "note": "shards run one after another on this GPU; on G GPUs each runs its chain at once, then a "
                          "reduce-scatter of the partials and the combine on N/G coefficients each"}
+        note("shard")
 
     # end to end from pageable host buffers (the fherma_run replacement)
     pcie_bytes = 12 * N * L
@@ -440,6 +453,7 @@ def main():
                       "floor_ms_at_measured": round(1e3 * floor_s, 3),
                       "pct_of_measured_floor": round(100 * floor_s / e2e_med, 1),
                       "bit_exact": bool(np.array_equal(c_host, c_dev))}
+        note("e2e")
 
     # batched: the same point 16 times, and the mixed batch of 64 uneven products
     del m, da, db, dc
@@ -449,12 +463,16 @@ def main():
     if "batch" in legs:
         tune = not args.no_autotune
         batch_same = batched(pmlib, O, 16, [(N, W)] * 16, SEED + 100, 16, 50, tune)
+        note("batched same point")
         rng = np.random.default_rng(SEED)
         mix = [(int(2 ** rng.integers(11, 17)), int(rng.integers(128, 2049))) for _ in range(64)]
         batch_mix = batched(pmlib, O, 64, mix, SEED + 200, 64, 50, tune)
         batch_mix["workload"] = "64 products, N log-uniform in 2^11..2^16, W uniform in [128, 2048], seed %d" % SEED
+        note("batched mix")
 
     serve = serving(pmlib, O, (200, 1000, 5000), args.serve_seconds, plain_serve) if "serve" in legs else None
+    if serve is not None:
+        note("serve")
 
     cpu = cpu_baseline(q, args.cpu_cases) if "cpu" in legs else None
     out = {
