@@ -19,9 +19,11 @@
 //     into the caller's buffer while the next is copied.
 // Chunks are contiguous coefficient ranges (the input layout is [N][L]), so
 // staging needs no gather; the column NTT pass simply starts after the last
-// input chunk instead of per chunk.
+// input chunk instead of per chunk. Points whose plans are single chunks (a at
+// most 1 MiB) skip the copy engines: the kernels read and write the pinned
+// staging buffers directly.
 //
-// PM_PROFILE=1 prints one phase line per call on stderr.
+// PM_PROFILE=1 prints one phase line per call on stderr (the chunked pipeline).
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -289,6 +291,35 @@ void State::run_host(const uint32_t* a, const uint32_t* b, uint32_t* c) {
             if (hi > lo) copy_stream(dst + lo, from + lo, (hi - lo) * 4);
         });
     };
+
+    // ---- small points: one chunk each way, nothing to overlap ----
+    // Every plan is a single chunk (a at most 1 MiB), so the pipeline's copy
+    // streams and cross-stream events would only add their latency. Both
+    // operands are staged in one team job; the RNS reads them from the pinned
+    // staging buffers and the CRT writes the result into pinned memory, so no
+    // copy engine is involved, and the five kernels run in order on the
+    // compute stream. Per call at N=4096/W=256: 0.062 ms, against 0.086-0.089
+    // with the copies on the compute stream and 0.107-0.119 through the
+    // chunked pipeline.
+    if (P.cin[0].size() == 1 && P.cin[1].size() == 1 && P.cout.size() == 1 && !P.profile) {
+        const size_t words = (size_t)host.N * L;
+        const size_t piece = (2 * host.N + T - 1) / T * L;  // whole coefficients over both operands
+        P.team->run(T, [&](int k) {
+            for (size_t lo = (size_t)k * piece, hi = std::min(2 * words, lo + piece); lo < hi;) {
+                const int op = lo >= words;
+                const size_t o = lo - op * words, n = std::min(hi, (op + 1) * words) - lo;
+                copy_stream(P.h_in[op] + o, src[op] + o, n * 4);
+                lo += n;
+            }
+        });
+        acquire_R(stream);
+        hip_check(enqueue_multiply(P.h_in[0], P.h_in[1], P.h_out, stream), "multiply");
+        release_R(stream);
+        hip_check(hipEventRecord(P.ev_out[0], stream), "record");
+        wait_event(P.ev_out[0]);
+        copy_job(c, P.h_out, words);
+        return;
+    }
 
     // ---- input: stage and H2D per chunk, every copy queued first ----
     // A cross-stream hipStreamWaitEvent costs 11-20 us of host time, about a
