@@ -8,8 +8,10 @@
 //   five phase launches over every request's tiles -> one D2H per request ->
 //   scatter (host team: pinned -> the caller's buffer),
 // on three streams (in, and two for compute and out) with up to `max_inflight`
-// batches at once, so one batch's transfers overlap another's compute. Latency is
-// measured from a request's arrival to the end of its scatter.
+// batches at once, so one batch's transfers overlap another's compute. A batch
+// with at most 2 MiB of operands skips the copies: its kernels read the pinned
+// staging and write the results there. Latency is measured from a request's
+// arrival to the end of its scatter.
 #include <hip/hip_runtime.h>
 #include <sys/resource.h>
 
@@ -141,6 +143,12 @@ constexpr size_t kCopyChunks[] = {1u << 20, 3u << 20, 16u << 20};
 constexpr size_t kOutChunks[] = {1u << 20, 5u << 20};
 constexpr size_t kOutTail = 512u << 10;
 constexpr size_t kOutMark = 1u << 20;
+
+// Batches with at most this many bytes of operands skip the copy engines: the
+// kernels read the operands from the pinned staging arena and the CRT writes
+// the results into it. A lone N=8192/W=438 request then waits for no
+// copy-engine or cross-stream hand-off.
+constexpr size_t kZeroCopyBytes = 2u << 20;
 
 }  // namespace
 
@@ -452,30 +460,56 @@ std::vector<Request*> Server::take_locked() {
 void Server::launch(Slot& S) {
     unsigned char* dev = static_cast<unsigned char*>(d_arena);
     Runner& R = S.run;
+    size_t in_bytes = 0;
+    for (const Request* r : S.reqs) in_bytes += 2 * cat.io_words(r->point) * 4;
+    const bool zero_copy = in_bytes <= kZeroCopyBytes;
     R.prods.assign(S.reqs.size(), BatchProduct{});
     for (size_t i = 0; i < S.reqs.size(); ++i) {
         const Request* r = S.reqs[i];
         BatchProduct& P = R.prods[i];
         P.D = cat.dev[r->point];
         uint32_t* base = reinterpret_cast<uint32_t*>(dev + r->dev_off);
-        P.a = base;
-        P.b = base + io_bytes[r->point] / 4;
-        P.c = base;  // the CRT phase runs after every RNS tile has read a
+        // a | padding | b, in device memory or, for a zero-copy batch, in the
+        // pinned arena (the same layout); the result goes over a: the CRT
+        // phase runs after every RNS tile has read a
+        uint32_t* io = zero_copy ? reinterpret_cast<uint32_t*>(h_arena + r->pin_off) : base;
+        P.a = io;
+        P.b = io + io_bytes[r->point] / 4;
+        P.c = io;
         P.R0 = base + 2 * io_bytes[r->point] / 4;
         P.R1 = P.R0 + res_bytes[r->point] / 4;
     }
     R.plan(cfg);
+    // A zero-copy batch whose groups all run the single-product kernels (plan
+    // and pointers as kernel arguments) reads no tables and puts nothing on
+    // the input stream, so it needs no cross-stream hand-off either (each
+    // costs 10-20 us of host time).
+    bool all_single = R.cfg.blocks_per_cu <= 0;
+    for (int g : R.single) all_single = all_single && g >= 0;
+    const bool via_input_stream = !(zero_copy && all_single);
     // The tile lists go first on the input stream, ahead of the operands: on
     // the compute stream their small copy would wait behind the copy engines'
     // queue of later batches' operands. The slot's tile buffers are free: a
     // slot is reused only after its batch retired.
-    R.upload(s_in, true);
+    if (via_input_stream) R.upload(s_in, true);
     // Stage each request's a and b into pinned memory and copy them in, chunk
     // by chunk: a request's a|b region (a, its padding to 256 bytes, b) is
     // contiguous in pinned and in device memory alike.
     std::vector<Piece> pieces;
     const int T = stage_team->size();
+    if (zero_copy) {
+        // stage every operand in one team job; the kernels read them in place
+        for (Request* r : S.reqs) {
+            const size_t n = cat.io_words(r->point) * 4, ib = io_bytes[r->point];
+            unsigned char* h = h_arena + r->pin_off;
+            const size_t piece = std::max<size_t>(64u << 10, (2 * n + T - 1) / T);
+            add_pieces(pieces, h, r->a, n, piece);
+            add_pieces(pieces, h + ib, r->b, n, piece);
+        }
+        stage_team->run((int)pieces.size(), [&](int i) { copy_stream(pieces[i].dst, pieces[i].src, pieces[i].n); });
+    }
     for (Request* r : S.reqs) {
+        if (zero_copy) break;
         const size_t n = cat.io_words(r->point) * 4, ib = io_bytes[r->point], span = ib + n;
         unsigned char* h = h_arena + r->pin_off;
         const bool first = r == S.reqs.front();
@@ -496,13 +530,33 @@ void Server::launch(Slot& S) {
         }
     }
     S.t_staged = now_s();
-    hip_check(hipEventRecord(S.ev_in, s_in), "record");
     hipStream_t sc = s_comp[batch_seq++ % 2];
-    hip_check(hipStreamWaitEvent(sc, S.ev_in, 0), "wait");
+    if (via_input_stream) {
+        hip_check(hipEventRecord(S.ev_in, s_in), "record");
+        hip_check(hipStreamWaitEvent(sc, S.ev_in, 0), "wait");
+    }
     hip_check(R.launch(sc), "batch launch");
     S.out.clear();
     S.out_end.clear();
     S.nmark = 0;
+    if (zero_copy) {
+        // the results are in the pinned arena once the batch's kernels end
+        const int Ts = scatter_team->size();
+        for (Request* r : S.reqs) {
+            const size_t n = cat.io_words(r->point) * 4;
+            add_pieces(S.out, r->c, h_arena + r->pin_off, n, std::max<size_t>(64u << 10, (n + Ts - 1) / Ts));
+        }
+        if (S.nmark == S.mark_ev.size()) {
+            hipEvent_t e = nullptr;
+            hip_check(hipEventCreateWithFlags(&e, hipEventDisableTiming), "event");
+            S.mark_ev.push_back(e);
+        }
+        hip_check(hipEventRecord(S.mark_ev[S.nmark++], sc), "record");
+        S.out_end.push_back(S.out.size());
+        hip_check(hipEventRecord(S.ev_out, sc), "record");
+        S.t_issued = now_s();
+        return;
+    }
     const int Ts = scatter_team->size();
     size_t pending = 0;
     auto mark = [&] {

@@ -392,6 +392,30 @@ def test_distinct_states_on_threads(O, pm):
     assert not errors, errors
 
 
+@pytest.mark.parametrize("N,W", [(16384, 256), (16384, 288), (8192, 438), (1, 8), (2048, 868)])
+def test_host_small_point_path(O, pm, N, W):
+    """pm_run at points whose chunk plans are single chunks (the whole output
+    within its first chunk: N L 4 up to about 0.5 MiB), where the kernels read
+    and write the pinned staging buffers directly on one stream. 16384/256 is
+    the largest such point at W = 256 (exactly 0.5 MiB); 16384/288 (L = 9) is
+    just above it and takes the chunked pipeline. Consecutive calls with new
+    operands, and a device-resident call between them on the same state, must
+    not see each other's data."""
+    q, L, m, P = _setup(O, pm, N, W)
+    print("small-point path N=%d W=%d K=%d" % (N, W, m.K), flush=True)
+    with m:
+        for s, kind in enumerate(["mixed", "max", "uniform"]):
+            a, b = O.operands(kind, N, W, q, 1700 + s + N)
+            got = m.run(a, b)
+            assert np.array_equal(got, P.multiply(a, b)), (N, W, kind)
+            if s == 0:
+                da, db = pm.DeviceBuffer.from_array(b), pm.DeviceBuffer.from_array(a)
+                dc = pm.DeviceBuffer(N * L * 4)
+                m.run_device(da, db, dc)
+                m.sync()
+                assert np.array_equal(dc.download((N, L)), P.multiply(b, a)), (N, W, "device")
+
+
 @pytest.mark.parametrize("env", [
     {"PM_IN_CHUNKS": "0.05,0.2", "PM_IN_TAIL": "0.03,0.01", "PM_IN_CHUNKS_B": "0.07", "PM_IN_TAIL_B": "0.02",
      "PM_OUT_CHUNKS": "0.1", "PM_OUT_TAIL": "0.02"},
