@@ -129,7 +129,7 @@ void State::upload_plan() {
     D.delta = H.delta;
     D.p = reinterpret_cast<const uint32_t*>(at(o_p));
     D.pinv = reinterpret_cast<const uint32_t*>(at(o_pinv));
-    D.tw = reinterpret_cast<const uint2*>(at(o_tw));
+    D.tw = reinterpret_cast<const uint32_t*>(at(o_tw));
     D.tw_stride = H.N;
     D.rns_frag = reinterpret_cast<const v4i*>(at(o_rf));
     D.rns_bias = reinterpret_cast<const uint64_t*>(at(o_rb));

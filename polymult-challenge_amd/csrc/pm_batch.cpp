@@ -116,7 +116,7 @@ void Catalog::build(uint32_t count, const uint32_t* N, const uint32_t* W, const 
         D.delta = H.delta;
         D.p = reinterpret_cast<const uint32_t*>(shared8);
         D.pinv = reinterpret_cast<const uint32_t*>(shared8 + pbytes);
-        D.tw = reinterpret_cast<const uint2*>(shared8 + twoff);
+        D.tw = reinterpret_cast<const uint32_t*>(shared8 + twoff);
         D.tw_stride = kSharedN;
         D.rns_frag = reinterpret_cast<const v4i*>(at(offs[i].rf));
         D.rns_bias = reinterpret_cast<const uint64_t*>(at(offs[i].rb));

@@ -21,7 +21,7 @@ struct DevPlan {
     uint64_t delta;
     const uint32_t* p;     // [K]
     const uint32_t* pinv;  // [K]
-    const uint2* tw;       // [K][tw_stride] Shoup pairs of psi^bitrev(k), k < N
+    const uint32_t* tw;    // [K][tw_stride] Shoup companions w' of w = psi^bitrev(k), k < N
     uint32_t tw_stride;    // N for a single point; 65536 for the shared batch table
     const v4i* rns_frag;   // [rns_rt][rns_ks][64]
     const uint64_t* rns_bias;

@@ -54,6 +54,26 @@ __device__ __forceinline__ v16i mfma_i8(v4i a, v4i b, v16i c) {
 // inverse uses the mirrored entry tw[2^(s0+ls+1) - 1 - (line 2^ls + i)].
 // ---------------------------------------------------------------------------
 
+// Twiddle sources. The tables in global memory hold only the Shoup companion
+// w' = floor(w 2^32 / p) of each twiddle w (4 bytes, half the pairs' traffic);
+// w = mulhi(w', p) + 1, exact for every w != 0 (all twiddles are powers of
+// psi): w' p = w 2^32 - r with r = w 2^32 mod p in [1, p), so w' p lies in
+// [(w - 1) 2^32, w 2^32). The column pass stages whole pairs in LDS.
+struct TwComp {
+    const u32* t;
+    u32 p;
+    __device__ __forceinline__ TwComp operator+(u32 o) const { return {t + o, p}; }
+    __device__ __forceinline__ uint2 operator[](int i) const {
+        const u32 c = t[i];
+        return make_uint2(__umulhi(c, p) + 1u, c);
+    }
+};
+struct TwPair {
+    const uint2* t;
+    __device__ __forceinline__ TwPair operator+(u32 o) const { return {t + o}; }
+    __device__ __forceinline__ uint2 operator[](int i) const { return t[i]; }
+};
+
 template <int NB>
 struct Line {
     static constexpr int LE = NB >= 4 ? 4 : NB;
@@ -63,8 +83,8 @@ struct Line {
     static constexpr int R2 = NB - LE;
 };
 
-template <int NB, int NA>
-__device__ __forceinline__ void fwd_pass1(u32 (&x)[NA][Line<NB>::E], const uint2* __restrict__ tab, int s0,
+template <int NB, int NA, class TW>
+__device__ __forceinline__ void fwd_pass1(u32 (&x)[NA][Line<NB>::E], const TW tab, int s0,
                                           u32 line, u32 p) {
     constexpr int E = Line<NB>::E, R1 = Line<NB>::R1;
 #pragma unroll
@@ -72,7 +92,7 @@ __device__ __forceinline__ void fwd_pass1(u32 (&x)[NA][Line<NB>::E], const uint2
         const int half = E >> (st + 1);
         const bool LAZY = st < R1 - 1;  // the pass's last stage leaves every output reduced
         // the stage's first entry as a pointer, so b becomes an immediate offset
-        const uint2* ts = tab + ((1u << (s0 + st)) + (line << st));
+        const TW ts = tab + ((1u << (s0 + st)) + (line << st));
 #pragma unroll
         for (int b = 0; b < (1 << st); ++b) {
             const uint2 w = ts[b];
@@ -90,8 +110,8 @@ __device__ __forceinline__ void fwd_pass1(u32 (&x)[NA][Line<NB>::E], const uint2
     }
 }
 
-template <int NB, int NA>
-__device__ __forceinline__ void fwd_pass2(u32 (&x)[NA][Line<NB>::E], const uint2* __restrict__ tab, int s0,
+template <int NB, int NA, class TW>
+__device__ __forceinline__ void fwd_pass2(u32 (&x)[NA][Line<NB>::E], const TW tab, int s0,
                                           u32 line, int t, u32 p) {
     constexpr int E = Line<NB>::E, R1 = Line<NB>::R1, R2 = Line<NB>::R2;
 #pragma unroll
@@ -100,7 +120,7 @@ __device__ __forceinline__ void fwd_pass2(u32 (&x)[NA][Line<NB>::E], const uint2
         const int half = 1 << (R2 - 1 - st);
         const bool LAZY = st < R2 - 1;
         const int nblk = E / (2 * half);
-        const uint2* ts = tab + ((1u << (s0 + ls)) + (line << ls) + (u32)(t * nblk));
+        const TW ts = tab + ((1u << (s0 + ls)) + (line << ls) + (u32)(t * nblk));
 #pragma unroll
         for (int b = 0; b < nblk; ++b) {
             const uint2 w = ts[b];
@@ -118,8 +138,8 @@ __device__ __forceinline__ void fwd_pass2(u32 (&x)[NA][Line<NB>::E], const uint2
     }
 }
 
-template <int NB>
-__device__ __forceinline__ void inv_pass2(u32 (&x)[Line<NB>::E], const uint2* __restrict__ tab, int s0, u32 line,
+template <int NB, class TW>
+__device__ __forceinline__ void inv_pass2(u32 (&x)[Line<NB>::E], const TW tab, int s0, u32 line,
                                           int t, u32 p) {
     constexpr int E = Line<NB>::E, R1 = Line<NB>::R1, R2 = Line<NB>::R2;
 #pragma unroll
@@ -129,7 +149,7 @@ __device__ __forceinline__ void inv_pass2(u32 (&x)[Line<NB>::E], const uint2* __
         const int nblk = E / (2 * half);
         // entry (2^(s0+ls+1) - 1 - i) for i = (line << ls) + t nblk + b: the
         // last block's entry as a pointer, b counted back from it
-        const uint2* ts = tab + ((2u << (s0 + ls)) - 1u - ((line << ls) + (u32)(t * nblk + nblk - 1)));
+        const TW ts = tab + ((2u << (s0 + ls)) - 1u - ((line << ls) + (u32)(t * nblk + nblk - 1)));
 #pragma unroll
         for (int b = 0; b < nblk; ++b) {
             const uint2 m = ts[nblk - 1 - b];
@@ -139,14 +159,14 @@ __device__ __forceinline__ void inv_pass2(u32 (&x)[Line<NB>::E], const uint2* __
     }
 }
 
-template <int NB>
-__device__ __forceinline__ void inv_pass1(u32 (&x)[Line<NB>::E], const uint2* __restrict__ tab, int s0, u32 line,
+template <int NB, class TW>
+__device__ __forceinline__ void inv_pass1(u32 (&x)[Line<NB>::E], const TW tab, int s0, u32 line,
                                           u32 p) {
     constexpr int E = Line<NB>::E, R1 = Line<NB>::R1;
 #pragma unroll
     for (int st = R1 - 1; st >= 0; --st) {
         const int half = E >> (st + 1);
-        const uint2* ts = tab + ((2u << (s0 + st)) - 1u - ((line << st) + (u32)((1 << st) - 1)));
+        const TW ts = tab + ((2u << (s0 + st)) - 1u - ((line << st) + (u32)((1 << st) - 1)));
 #pragma unroll
         for (int b = 0; b < (1 << st); ++b) {
             const uint2 m = ts[(1 << st) - 1 - b];
@@ -184,7 +204,7 @@ __device__ __forceinline__ void col_tile(const DevPlan& D, u32* __restrict__ R, 
     const bool on = tid < tc * T;
     const int c = tid % tc, t = tid / tc;
     const u32 p = D.p[j];
-    const uint2* tab = D.tw + (size_t)j * D.tw_stride;
+    const TwComp tab{D.tw + (size_t)j * D.tw_stride, p};
     // The tile's base is the same for the whole block; each access adds a
     // 32-bit byte offset to it (the prime's vector is below 2^32 bytes), one
     // 64-bit add per access instead of a sign extension, a 64-bit shift and
@@ -225,7 +245,7 @@ __device__ __forceinline__ void col_tile(const DevPlan& D, u32* __restrict__ R, 
             if (on) {
 #pragma unroll
                 for (int k = 0; k < E; ++k) x[0][k] = lds[(t * E + k) * tc + c];
-                fwd_pass2<NB, 1>(x, ltw, 0, 0, t, p);
+                fwd_pass2<NB, 1>(x, TwPair{ltw}, 0, 0, t, p);
 #pragma unroll
                 for (int k = 0; k < E; ++k) blocked(k) = x[0][k];
             }
@@ -242,7 +262,7 @@ __device__ __forceinline__ void col_tile(const DevPlan& D, u32* __restrict__ R, 
             stage_twiddles();
             __syncthreads();
             if (on) {
-                inv_pass2<NB>(x[0], ltw, 0, 0, t, p);
+                inv_pass2<NB>(x[0], TwPair{ltw}, 0, 0, t, p);
 #pragma unroll
                 for (int k = 0; k < E; ++k) lds[(t * E + k) * tc + c] = x[0][k];
             }
@@ -304,7 +324,7 @@ __device__ __forceinline__ void row_tile(const DevPlan& D, u32* __restrict__ RB,
         row = (u32)r < h ? base + (u32)r : nrows - 1u - (base + (u32)r - h);
     }
     const u32 p = D.p[j];
-    const uint2* tab = D.tw + (size_t)j * D.tw_stride;
+    const TwComp tab{D.tw + (size_t)j * D.tw_stride, p};
     const size_t off = (size_t)j * D.N + ((size_t)row << NB);
     u32* rb = RB + off;
     u32* lb = lds + r * RS;
@@ -466,7 +486,9 @@ __device__ __forceinline__ void row_tile(const DevPlan& D, u32* __restrict__ RB,
 }
 
 // At most 80 VGPRs (6 waves per SIMD): with one row buffer per row the LDS
-// allows that many resident blocks; 64 would spill.
+// allows that many resident blocks. The fused pass takes 77 with the twiddle
+// recovery; held to 7 waves it fits 62 without spills but measured 0.5% lower
+// in three-lane throughput at N=65536/W=1747.
 template <int NB, int MODE>
 __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(6, 8)))
 k_row(DevPlan D, u32* __restrict__ RB, const u32* __restrict__ RA, int rpb) {

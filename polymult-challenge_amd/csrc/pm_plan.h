@@ -32,8 +32,8 @@ struct HostPlan {
     std::vector<uint32_t> p, pinv;  // [K]; pinv = -p^-1 mod 2^32
     std::vector<uint32_t> psi;      // [K] primitive 2N-th roots
 
-    // NTT: tw[j][k] = (w, w') with w = psi_j^bitrev_logN(k) mod p_j and the
-    // Shoup companion w' = floor(w 2^32 / p_j). [K][N][2]
+    // NTT: tw[j][k] = w', the Shoup companion floor(w 2^32 / p_j) of
+    // w = psi_j^bitrev_logN(k) mod p_j; w = floor(w' p_j / 2^32) + 1. [K][N]
     std::vector<uint32_t> tw;
 
     // RNS GEMM (int8 MFMA, 32x32x32): A fragments in lane order,
@@ -76,7 +76,7 @@ void shared_primes(SharedPrimes& S, int count);
 // plan_point (the first K primes with sum log2 p >= 2W + logN + 3).
 void plan_point_shared(HostPlan& P, uint32_t N, uint32_t W, uint32_t L, const uint32_t* q, size_t q_limbs,
                        const SharedPrimes& S);
-// tw[j][k] (Shoup pairs) for j < K, k < 65536 over the shared primes.
+// tw[j][k] (Shoup companions, as HostPlan::tw) for j < K, k < 65536 over the shared primes.
 void build_shared_twiddles(const SharedPrimes& S, int K, std::vector<uint32_t>& tw);
 
 }  // namespace pm

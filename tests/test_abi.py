@@ -114,3 +114,28 @@ def test_struct_layouts_match_header(pm, tmp_path):
         assert int(got[cname]) == ctypes.sizeof(py), cname
         for f, _ in py._fields_:
             assert int(got["%s.%s" % (cname, f)]) == getattr(py, f).offset, (cname, f)
+
+
+def test_twiddle_companion_recovers_twiddle(O):
+    """The NTT tables hold only the Shoup companion w' = floor(w 2^32 / p) of
+    each twiddle; the kernels recover w = floor(w' p / 2^32) + 1. That holds
+    for every w in [1, p) (no twiddle is zero, all are powers of psi) and
+    fails for w = 0. Checked exhaustively for small primes, and for the 114
+    primes of the N=65536/W=1747 plan on w = 1, 2, p - 1 and 20,000 random w
+    each, plus the plan's own twiddles psi^k for k < 4096."""
+    def recover(w, p):
+        c = (w << 32) // p
+        return ((c * p) >> 32) + 1
+
+    for p in (3, 5, 17, 97, 257, 7681, 12289):
+        assert all(recover(w, p) == w for w in range(1, p)), p
+        assert recover(0, p) != 0
+    rng = np.random.default_rng(2026)
+    P = O.OraclePlan(65536, 1747, O.spec_q(65536, 1747))
+    for p, psi in zip(P.primes.tolist(), P.psi.tolist()):
+        ws = [1, 2, p - 1] + rng.integers(1, p, 20000).tolist()
+        assert all(recover(w, p) == w for w in ws), p
+        w = 1
+        for _ in range(4096):
+            assert recover(w, p) == w, (p, w)
+            w = w * psi % p
