@@ -24,7 +24,8 @@
 //   k_crt  = k_crt (solve.cu:380-501): y_j, v, z = sum y_j Q_j + T[v], fold, -q
 // The twiddle of CT stage s, block i is psi^bitrev(2^s + i) and the inverse
 // uses the mirrored entry; the table entries are the reference's tw(k)
-// (solve.cu:947-953), here as Shoup pairs rather than Montgomery forms.
+// (solve.cu:947-953), here used as Shoup pairs rather than Montgomery forms
+// and stored as the companions alone (see TwComp).
 #include "pm_kernels.h"
 
 #include <cstdlib>
