@@ -179,3 +179,28 @@ def test_serve_other_configs(O, pm, config):
     for i, (p, a, b) in enumerate(reqs):
         N, W, q = cat[p]
         assert np.array_equal(got[i], O.OraclePlan(N, W, q).multiply(a, b)), "request %d" % i
+
+
+def test_serve_zero_copy_threshold(O, pm):
+    """Batches with at most 2 MiB of operands run their kernels on the pinned
+    staging (no copies); larger ones go through the copy engines. 32768/256
+    (L = 8) has exactly 2 MiB of operands, 32768/288 (L = 9) 2.25 MiB, and
+    1024/256 is far below. Each is served alone, then all together (which
+    puts the small and the threshold request in one batch above 2 MiB), and
+    every result must equal the oracle's."""
+    cat = _catalog(O, [(32768, 256), (32768, 288), (1024, 256)])
+    plans = [O.OraclePlan(N, W, q) for N, W, q in cat]
+    reqs = []
+    for i in range(6):
+        p = i % len(cat)
+        N, W, q = cat[p]
+        reqs.append((p, *O.operands("mixed" if i < 3 else "max", N, W, q, 900 + i)))
+    with pm.Server(cat) as S:
+        for p, a, b in reqs:
+            print("zero-copy threshold, alone: point", cat[p][:2], flush=True)
+            c, _ = S.wait(S.submit(p, a, b))
+            assert np.array_equal(c, plans[p].multiply(a, b)), ("alone", cat[p][:2])
+        print("zero-copy threshold, together", flush=True)
+        tickets = [S.submit(p, a, b) for p, a, b in reqs]
+        for (p, a, b), t in zip(reqs, tickets):
+            assert np.array_equal(S.wait(t)[0], plans[p].multiply(a, b)), ("together", cat[p][:2])
