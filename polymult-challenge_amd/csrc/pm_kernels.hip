@@ -790,10 +790,12 @@ __device__ __forceinline__ void crt_finish(const DevPlan& D, GET get, u32* __res
         }
     }
     __syncthreads();
+    // Non-temporal: no kernel reads the result, so it need not displace the
+    // residues and twiddles in the caches.
     u32* dst = out + (size_t)n0 * L;
     for (int idx = tid; idx < cnt * L; idx += nthreads) {
         const int c = L == 1 ? idx : (int)__umulhi((u32)idx, D.magic_L);  // idx / L
-        dst[idx] = Z[(size_t)c * ZL + (idx - c * L)];
+        __builtin_nontemporal_store(Z[(size_t)c * ZL + (idx - c * L)], &dst[idx]);
     }
 }
 
@@ -956,10 +958,12 @@ k_crt_alias(DevPlan D, const u32* __restrict__ R, u32* __restrict__ out, const u
     __syncthreads();
     for (int c = tid; c < cnt; c += nthreads) crt_finish_inplace(D, sl + c * S);
     __syncthreads();
+    // non-temporal, as in crt_finish: no kernel reads the result (20.4-20.6 ->
+    // 18.9-19.1 us at 65536/1747, three-lane throughput unchanged)
     u32* dst = out + (size_t)n0 * L;
     for (int idx = tid; idx < cnt * L; idx += nthreads) {
         const int c = L == 1 ? idx : (int)__umulhi((u32)idx, D.magic_L);  // idx / L
-        dst[idx] = sl[c * S + 2 * (idx - c * L)];
+        __builtin_nontemporal_store(sl[c * S + 2 * (idx - c * L)], &dst[idx]);
     }
 }
 
