@@ -75,6 +75,28 @@ struct TwPair {
     __device__ __forceinline__ uint2 operator[](int i) const { return t[i]; }
 };
 
+// Residues are written once by each pass and read by the next kernel, which
+// may run on any XCD. Stores with the sc1 policy (device scope: through the
+// XCD's L2 to the last-level cache) leave the L2 no dirty lines to write back
+// and the next kernel finds the data in the last-level cache; non-temporal
+// stores would bypass that cache too (measured slower for the kernel after).
+// At 65536/1747, against plain stores (one box, alternating runs): the RNS
+// 25.9-26.3 -> 20.7-21.3 us (+0.7% three-lane throughput on its own), then
+// forward columns 30.0 -> 26.0-26.3, inverse columns 16.0 -> 14.5-14.8, fused
+// rows 41.2-41.8 -> 40.3-40.6 (+2% more). Non-temporal RNS stores ran 22.5 us
+// but slowed the column pass by 1.5 us, -1.5% overall.
+constexpr int kResidueStorePolicy = 16;  // gfx950 CPol: sc1
+
+struct ResidueStore {
+    __amdgpu_buffer_rsrc_t rsrc;
+    // over R[0 .. words): every store of a pass lands inside its [K][N] array
+    __device__ __forceinline__ ResidueStore(u32* R, size_t words)
+        : rsrc(__builtin_amdgcn_make_buffer_rsrc(R, (short)0, (int)(words * 4), 0x00020000)) {}
+    __device__ __forceinline__ void operator()(int byte_off, u32 v) const {
+        __builtin_amdgcn_raw_buffer_store_b32(v, rsrc, byte_off, 0, kResidueStorePolicy);
+    }
+};
+
 template <int NB>
 struct Line {
     static constexpr int LE = NB >= 4 ? 4 : NB;
@@ -218,6 +240,11 @@ __device__ __forceinline__ void col_tile(const DevPlan& D, u32* __restrict__ R, 
     const u32 blocked0 = (((u32)(t * E) << sh) + (u32)c) << 2;  // positions t E + k
     auto strided = [&](int k) -> u32& { return *reinterpret_cast<u32*>(base + strided0 + ((u32)(k * T) << (sh + 2))); };
     auto blocked = [&](int k) -> u32& { return *reinterpret_cast<u32*>(base + blocked0 + ((u32)k << (sh + 2))); };
+    // stores go through a buffer resource (byte offsets from R, same addresses)
+    const ResidueStore store(R, (size_t)D.K * D.N);
+    const int obase = (int)(((size_t)j * D.N + ((size_t)hi << (NB + rest)) + col0) * 4);
+    auto store_strided = [&](int k, u32 v) { store(obase + (int)(strided0 + ((u32)(k * T) << (sh + 2))), v); };
+    auto store_blocked = [&](int k, u32 v) { store(obase + (int)(blocked0 + ((u32)k << (sh + 2))), v); };
     u32 x[1][E];
     uint2* ltw = reinterpret_cast<uint2*>(lds + (1 << NB) * tc);
     auto stage_twiddles = [&] {
@@ -248,11 +275,11 @@ __device__ __forceinline__ void col_tile(const DevPlan& D, u32* __restrict__ R, 
                 for (int k = 0; k < E; ++k) x[0][k] = lds[(t * E + k) * tc + c];
                 fwd_pass2<NB, 1>(x, TwPair{ltw}, 0, 0, t, p);
 #pragma unroll
-                for (int k = 0; k < E; ++k) blocked(k) = x[0][k];
+                for (int k = 0; k < E; ++k) store_blocked(k, x[0][k]);
             }
         } else if (on) {
 #pragma unroll
-            for (int k = 0; k < E; ++k) strided(k) = x[0][k];
+            for (int k = 0; k < E; ++k) store_strided(k, x[0][k]);
         }
     } else {
         if constexpr (R2 > 0) {
@@ -279,7 +306,7 @@ __device__ __forceinline__ void col_tile(const DevPlan& D, u32* __restrict__ R, 
         if (on) {
             inv_pass1<NB>(x[0], tab, s0, hi, p);
 #pragma unroll
-            for (int k = 0; k < E; ++k) strided(k) = x[0][k];
+            for (int k = 0; k < E; ++k) store_strided(k, x[0][k]);
         }
     }
 }
@@ -329,6 +356,8 @@ __device__ __forceinline__ void row_tile(const DevPlan& D, u32* __restrict__ RB,
     const size_t off = (size_t)j * D.N + ((size_t)row << NB);
     u32* rb = RB + off;
     u32* lb = lds + r * RS;
+    const ResidueStore store(RB, (size_t)D.K * D.N);  // rb[i] is byte rboff + 4 i of it
+    const int rboff = (int)(off * 4);
     // padded LDS index of position t + k T (t < T, T a power of two up to 16:
     // (t + k T) / 16 = k T / 16) and of position 16 t + k (k < 16)
     static_assert(R2 == 0 || (E == 16 && T <= 16), "pad mapping assumes 16 values per thread");
@@ -386,7 +415,7 @@ __device__ __forceinline__ void row_tile(const DevPlan& D, u32* __restrict__ RB,
         if (on) {
             inv_pass1<NB>(y, tab, s0, row, p);
 #pragma unroll
-            for (int k = 0; k < E; ++k) rb[t + k * T] = y[k];
+            for (int k = 0; k < E; ++k) store(rboff + (int)((t + k * T) * 4), y[k]);
         }
     } else if constexpr (MODE == ROW_FUSED_PRE) {
         // b's forward rows; a's row is already in the NTT domain, each
@@ -433,7 +462,7 @@ __device__ __forceinline__ void row_tile(const DevPlan& D, u32* __restrict__ RB,
         if (on) {
             inv_pass1<NB>(y, tab, s0, row, p);
 #pragma unroll
-            for (int k = 0; k < E; ++k) rb[t + k * T] = y[k];
+            for (int k = 0; k < E; ++k) store(rboff + (int)((t + k * T) * 4), y[k]);
         }
     } else if constexpr (MODE == ROW_FWD) {
         u32 x[1][E];
@@ -453,11 +482,11 @@ __device__ __forceinline__ void row_tile(const DevPlan& D, u32* __restrict__ RB,
                 for (int k = 0; k < E; ++k) x[0][k] = lb[pc(k)];
                 fwd_pass2<NB, 1>(x, tab, s0, row, t, p);
 #pragma unroll
-                for (int k = 0; k < E; ++k) rb[t * E + k] = x[0][k];
+                for (int k = 0; k < E; ++k) store(rboff + (int)((t * E + k) * 4), x[0][k]);
             }
         } else if (on) {
 #pragma unroll
-            for (int k = 0; k < E; ++k) rb[t + k * T] = x[0][k];
+            for (int k = 0; k < E; ++k) store(rboff + (int)((t + k * T) * 4), x[0][k]);
         }
     } else {  // ROW_INV
         u32 y[E];
@@ -481,7 +510,7 @@ __device__ __forceinline__ void row_tile(const DevPlan& D, u32* __restrict__ RB,
         if (on) {
             inv_pass1<NB>(y, tab, s0, row, p);
 #pragma unroll
-            for (int k = 0; k < E; ++k) rb[t + k * T] = y[k];
+            for (int k = 0; k < E; ++k) store(rboff + (int)((t + k * T) * 4), y[k]);
         }
     }
 }
@@ -543,14 +572,6 @@ __device__ __forceinline__ long long slot_value(const v16i& acc, int g) {
 // ---------------------------------------------------------------------------
 constexpr int kRnsNarrowL = 127;
 
-// Cache policy of the full tiles' residue stores: sc1 (gfx950's CPol bit 4).
-// Measured at 65536/1747 against plain stores (one box, four alternating
-// runs): the RNS 20.7-21.3 us against 25.9-26.3, three-lane throughput
-// 9,792-9,836 against 9,714-9,783 mults/s; the forward column pass, which
-// reads the residues next, 0.3-0.8 us slower. Non-temporal stores (nt) made
-// the RNS 22.5 us but the column pass 1.5 us slower and cost 1.5% overall.
-constexpr int kRnsStorePolicy = 16;
-
 template <int NCT, bool WIDE = false>
 __device__ __forceinline__ void rns_tile(const DevPlan& D, const u32* __restrict__ a, u32* __restrict__ R, u32 n0,
                                          u32 count, int vec, u32* lds32, int tid, int nthreads) {
@@ -607,9 +628,8 @@ __device__ __forceinline__ void rns_tile(const DevPlan& D, const u32* __restrict
         }
     };
     // full tiles store through a buffer resource over R (every offset is in
-    // range: j < K, n0 + TN <= N), with the cache policy above
-    const __amdgpu_buffer_rsrc_t rsrc =
-        __builtin_amdgcn_make_buffer_rsrc(R, (short)0, (int)((size_t)K * N * 4), 0x00020000);
+    // range: j < K, n0 + TN <= N)
+    const ResidueStore store(R, (size_t)K * N);
     for (int rt = wave; rt < D.rns_rt; rt += nw) {
         v16i acc[NCT];
 #pragma unroll
@@ -625,9 +645,7 @@ __device__ __forceinline__ void rns_tile(const DevPlan& D, const u32* __restrict
             if (cnt == TN) {
                 const int off = (int)(((size_t)j * N + n0 + col) * 4);
 #pragma unroll
-                for (int ct = 0; ct < NCT; ++ct)
-                    __builtin_amdgcn_raw_buffer_store_b32(reduce(acc[ct], g, bias, k4.x, k4.y), rsrc, off + 128 * ct, 0,
-                                                          kRnsStorePolicy);
+                for (int ct = 0; ct < NCT; ++ct) store(off + 128 * ct, reduce(acc[ct], g, bias, k4.x, k4.y));
             } else {
 #pragma unroll
                 for (int ct = 0; ct < NCT; ++ct)
