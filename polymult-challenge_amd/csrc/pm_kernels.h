@@ -66,22 +66,30 @@ LaunchCfg choose_launch(const DevPlan& D, size_t max_lds, size_t device_lds);
 
 // R_i[j][n] = a_i[n] mod p_j for nops (1 or 2) operands, for the coefficients
 // n in [first, first + count) (count 0: to the end). a_i and R_i are the
-// whole arrays; `first` must be a multiple of 128.
+// whole arrays; `first` must be a multiple of 128. lcw > 0: a_i is staged in
+// column windows of 2^lcw columns (two-pass plans; staging position i holds
+// coefficient n = r 2^w2 + w 2^lcw + c for i = (w 2^w1 + r) 2^lcw + c), and
+// [first, first + count) are staging positions, `first` on a window.
 hipError_t launch_rns(const DevPlan& D, const LaunchCfg& C, const uint32_t* a0, const uint32_t* a1,
-                      uint32_t* R0, uint32_t* R1, int nops, hipStream_t s, uint32_t first = 0, uint32_t count = 0);
+                      uint32_t* R0, uint32_t* R1, int nops, hipStream_t s, uint32_t first = 0, uint32_t count = 0,
+                      int lcw = 0);
 // The strided passes (every pass but the row pass: stages 0 .. logN - w2 - 1,
 // or their inverse in reverse order) on nops residue arrays.
+// col_count > 0 (two-pass plans): only columns [col_first, col_first +
+// col_count) of the 2^w2, both multiples of the column tile.
 hipError_t launch_col(const DevPlan& D, const LaunchCfg& C, uint32_t* R0, uint32_t* R1, int nops, bool inverse,
-                      hipStream_t s);
+                      hipStream_t s, uint32_t col_first = 0, uint32_t col_count = 0);
 // Row pass. ROW_FUSED: RB <- inv(fwd(RB) * fwd(RA)) per row (RA untouched,
 // the product carries 2^-32). ROW_FUSED_PRE: the same with RA already through
 // ROW_FWD (so a's rows can run before b is in). ROW_FWD / ROW_INV: RB only.
 hipError_t launch_row(const DevPlan& D, const LaunchCfg& C, uint32_t* RB, const uint32_t* RA, RowMode mode,
                       hipStream_t s);
 // out[n][L] = CRT of R[j][n] (scaled by G) mod q, for n in [first, first + count).
-// With C.crt_split, `part` is the (L + 3) N int64 slot buffer.
+// With C.crt_split, `part` is the (L + 3) N int64 slot buffer. lcw > 0 (the
+// aliased CRT only): out is in column-window order as launch_rns's input, and
+// [first, first + count) are its positions.
 hipError_t launch_crt(const DevPlan& D, const LaunchCfg& C, const uint32_t* R, uint32_t* out, const uint2* G,
-                      hipStream_t s, uint32_t first = 0, uint32_t count = 0, long long* part = nullptr);
+                      hipStream_t s, uint32_t first = 0, uint32_t count = 0, long long* part = nullptr, int lcw = 0);
 
 // Prime shards: D a shard's view (its primes, whole k-steps). P[slot][n]
 // int64, L + 3 slots; the combine sums `shards` such arrays (shard_stride

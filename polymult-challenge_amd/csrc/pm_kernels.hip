@@ -97,6 +97,20 @@ struct ResidueStore {
     }
 };
 
+// Column windows (two-pass plans). A host call can stage an operand, or take
+// its result, one window of CW = 2^lcw columns at a time, so that each
+// window's column pass can run as soon as that window is in, or its result can
+// leave as soon as that window's inverse column pass is done. Staging position
+// i = (w 2^w1 + r) CW + c, with c < CW, row r < 2^w1 and window w, holds
+// coefficient n = r 2^w2 + w CW + c: a window is contiguous in staging, and
+// each of its rows is one run of CW coefficients of the [N][L] array.
+// lcw = 0: no window, n = i.
+__device__ __forceinline__ u32 window_pos(const DevPlan& D, u32 i, int lcw) {
+    if (lcw == 0) return i;
+    const u32 r = (i >> lcw) & ((1u << D.w1) - 1u);
+    return (r << D.w2) | ((i >> (lcw + D.w1)) << lcw) | (i & ((1u << lcw) - 1u));
+}
+
 template <int NB>
 struct Line {
     static constexpr int LE = NB >= 4 ? 4 : NB;
@@ -572,9 +586,11 @@ __device__ __forceinline__ long long slot_value(const v16i& acc, int g) {
 // ---------------------------------------------------------------------------
 constexpr int kRnsNarrowL = 127;
 
+// Coefficients [n0, n0 + TN) of a (count in all) into residue positions
+// [nst, nst + TN) of R (nst = n0 but for a column window).
 template <int NCT, bool WIDE = false>
 __device__ __forceinline__ void rns_tile(const DevPlan& D, const u32* __restrict__ a, u32* __restrict__ R, u32 n0,
-                                         u32 count, int vec, u32* lds32, int tid, int nthreads) {
+                                         u32 count, int vec, u32* lds32, int tid, int nthreads, u32 nst) {
     constexpr int TN = 32 * NCT;
     const int L = D.L, K = D.K;
     const u32 N = D.N;
@@ -628,7 +644,7 @@ __device__ __forceinline__ void rns_tile(const DevPlan& D, const u32* __restrict
         }
     };
     // full tiles store through a buffer resource over R (every offset is in
-    // range: j < K, n0 + TN <= N)
+    // range: j < K, nst + TN <= N)
     const ResidueStore store(R, (size_t)K * N);
     for (int rt = wave; rt < D.rns_rt; rt += nw) {
         v16i acc[NCT];
@@ -641,9 +657,9 @@ __device__ __forceinline__ void rns_tile(const DevPlan& D, const u32* __restrict
             const uint4 k4 = kc[j];
             if (j >= K) continue;
             const u64 bias = (u64)k4.z | ((u64)k4.w << 32);
-            u32* row = R + (size_t)j * N + n0 + col;
+            u32* row = R + (size_t)j * N + nst + col;
             if (cnt == TN) {
-                const int off = (int)(((size_t)j * N + n0 + col) * 4);
+                const int off = (int)(((size_t)j * N + nst + col) * 4);
 #pragma unroll
                 for (int ct = 0; ct < NCT; ++ct) store(off + 128 * ct, reduce(acc[ct], g, bias, k4.x, k4.y));
             } else {
@@ -657,10 +673,11 @@ __device__ __forceinline__ void rns_tile(const DevPlan& D, const u32* __restrict
 
 template <int NCT, bool WIDE>
 __global__ void __launch_bounds__(512) k_rns(DevPlan D, const u32* __restrict__ a0, const u32* __restrict__ a1,
-                                              u32* __restrict__ R0, u32* __restrict__ R1, u32 count, int vec) {
+                                              u32* __restrict__ R0, u32* __restrict__ R1, u32 count, int vec, int lcw) {
     extern __shared__ __align__(16) u32 lds32[];
-    rns_tile<NCT, WIDE>(D, blockIdx.y ? a1 : a0, blockIdx.y ? R1 : R0, blockIdx.x * (u32)(32 * NCT), count, vec,
-                        lds32, (int)threadIdx.x, (int)blockDim.x);
+    const u32 n0 = blockIdx.x * (u32)(32 * NCT);
+    rns_tile<NCT, WIDE>(D, blockIdx.y ? a1 : a0, blockIdx.y ? R1 : R0, n0, count, vec, lds32, (int)threadIdx.x,
+                        (int)blockDim.x, window_pos(D, n0, lcw));
 }
 
 // ---------------------------------------------------------------------------
@@ -937,7 +954,7 @@ constexpr int kCrtAliasPF = 4;
 template <int NCT, int MINW>  // NCT 1 or 2; MINW: waves per SIMD the registers must allow
 __global__ void __launch_bounds__(NCT == 1 ? 256 : 512) __attribute__((amdgpu_waves_per_eu(MINW, 8)))
 k_crt_alias(DevPlan D, const u32* __restrict__ R, u32* __restrict__ out, const uint2* __restrict__ G, u32 count, int ys,
-            int kc_off) {
+            int kc_off, int lcw) {
     extern __shared__ __align__(16) unsigned char lds8[];
     constexpr int TN = 32 * NCT, RTW = NCT == 1 ? 2 : 1;  // coefficients per block, row tiles per wave
     const int tid = (int)threadIdx.x, nthreads = (int)blockDim.x;
@@ -948,7 +965,7 @@ k_crt_alias(DevPlan D, const u32* __restrict__ R, u32* __restrict__ out, const u
     uint4* kc = reinterpret_cast<uint4*>(lds8 + kc_off);      // [8 crt_ks]
     crt_consts(D, G, kc, tid, nthreads);
     __syncthreads();
-    crt_y<TN>(D, R, kc, n0, cnt, ys, Y, tid, nthreads);
+    crt_y<TN>(D, R, kc, window_pos(D, n0, lcw), cnt, ys, Y, tid, nthreads);
     __syncthreads();
     const int lane = tid & 63, wave = tid >> 6, nw = nthreads >> 6;
     const int half = lane >> 5, col = lane & 31;
@@ -1116,7 +1133,7 @@ __device__ __forceinline__ void batch_task(const BatchProduct& P, u32 x, u32 y, 
     // scalar registers) rather than re-read through the product pointer.
     const DevPlan D = P.D;
     if constexpr (PH == PH_RNS) {
-        rns_tile<kBatchRnsTile / 32>(D, y ? P.b : P.a, y ? P.R1 : P.R0, x, D.N, 1, lds, tid, kBatchThreads[PH]);
+        rns_tile<kBatchRnsTile / 32>(D, y ? P.b : P.a, y ? P.R1 : P.R0, x, D.N, 1, lds, tid, kBatchThreads[PH], x);
     } else if constexpr (PH == PH_COLF) {
         col_tile_nb<false>(D.w1, D, (y >> 16) ? P.R1 : P.R0, (int)(y & 0xFFFF), (int)x, P.col_tc, lds, tid);
     } else if constexpr (PH == PH_ROW) {
@@ -1311,35 +1328,65 @@ LaunchCfg choose_launch(const DevPlan& D, size_t max_lds, size_t device_lds) {
     return C;
 }
 
+// A column window's staging range starts on a window and, within the
+// launch, keeps every tile inside one run of CW coefficients; the residues of
+// staging position `first` then sit first / 2^w1 positions on.
+static bool window_ok(const DevPlan& D, int lcw, int tile, u32 first, u32 count) {
+    if (lcw == 0) return true;
+    const u32 win = 1u << (lcw + D.w1);
+    return D.npass == 2 && lcw <= D.w2 && (1 << lcw) >= tile && first % win == 0 && count % (u32)tile == 0;
+}
+static u32 residue_first(const DevPlan& D, int lcw, u32 first) { return lcw ? first >> D.w1 : first; }
+
 hipError_t launch_rns(const DevPlan& D, const LaunchCfg& C, const u32* a0, const u32* a1, u32* R0, u32* R1, int nops,
-                      hipStream_t s, u32 first, u32 count) {
+                      hipStream_t s, u32 first, u32 count, int lcw) {
     if (count == 0) count = D.N - first;
-    if (first + count > D.N) return hipErrorInvalidValue;
+    if (first + count > D.N || !window_ok(D, lcw, C.rns_tn, first, count)) return hipErrorInvalidValue;
     const dim3 grid((count + C.rns_tn - 1) / C.rns_tn, nops), block(C.rns_threads);
     const size_t L = (size_t)D.L;
+    const u32 rfirst = residue_first(D, lcw, first);
     a0 += first * L;
-    R0 += first;
+    R0 += rfirst;
     if (nops > 1) {
         a1 += first * L;
-        R1 += first;
+        R1 += rfirst;
     }
     const int vec = ((uintptr_t)a0 % 16 == 0) && (nops < 2 || (uintptr_t)a1 % 16 == 0);
     if (D.L > kRnsNarrowL) {  // 64-bit epilogue; such rows fill the LDS at 32 coefficients
         if (C.rns_tn != 32) return hipErrorInvalidValue;
-        hipLaunchKernelGGL((k_rns<1, true>), grid, block, C.rns_lds, s, D, a0, a1, R0, R1, count, vec);
+        hipLaunchKernelGGL((k_rns<1, true>), grid, block, C.rns_lds, s, D, a0, a1, R0, R1, count, vec, lcw);
         return hipGetLastError();
     }
     switch (C.rns_tn) {
-        case 256: hipLaunchKernelGGL((k_rns<8, false>), grid, block, C.rns_lds, s, D, a0, a1, R0, R1, count, vec); break;
-        case 128: hipLaunchKernelGGL((k_rns<4, false>), grid, block, C.rns_lds, s, D, a0, a1, R0, R1, count, vec); break;
-        case 64: hipLaunchKernelGGL((k_rns<2, false>), grid, block, C.rns_lds, s, D, a0, a1, R0, R1, count, vec); break;
-        case 32: hipLaunchKernelGGL((k_rns<1, false>), grid, block, C.rns_lds, s, D, a0, a1, R0, R1, count, vec); break;
+        case 256: hipLaunchKernelGGL((k_rns<8, false>), grid, block, C.rns_lds, s, D, a0, a1, R0, R1, count, vec, lcw); break;
+        case 128: hipLaunchKernelGGL((k_rns<4, false>), grid, block, C.rns_lds, s, D, a0, a1, R0, R1, count, vec, lcw); break;
+        case 64: hipLaunchKernelGGL((k_rns<2, false>), grid, block, C.rns_lds, s, D, a0, a1, R0, R1, count, vec, lcw); break;
+        case 32: hipLaunchKernelGGL((k_rns<1, false>), grid, block, C.rns_lds, s, D, a0, a1, R0, R1, count, vec, lcw); break;
         default: return hipErrorInvalidValue;
     }
     return hipGetLastError();
 }
 
-hipError_t launch_col(const DevPlan& D, const LaunchCfg& C, u32* R0, u32* R1, int nops, bool inverse, hipStream_t s) {
+hipError_t launch_col(const DevPlan& D, const LaunchCfg& C, u32* R0, u32* R1, int nops, bool inverse, hipStream_t s,
+                      u32 col_first, u32 col_count) {
+    if (col_count) {
+        // columns [col_first, col_first + col_count) of a two-pass plan: the
+        // tiles of the one strided pass that cover them (its twiddles depend on
+        // the stage and the row, not on the column)
+        const u32 tc = (u32)std::min<u32>((u32)C.col_tc, 1u << D.w2);
+        if (D.npass != 2 || col_first % tc || col_count % tc || col_first + col_count > (1u << D.w2))
+            return hipErrorInvalidValue;
+        const int nb = D.pw[0], T = line_threads(nb);
+        const u32 tiles = col_count / tc;
+        int tiles_log = 0;
+        while ((1u << tiles_log) < tiles) ++tiles_log;
+        R0 += col_first;
+        if (nops > 1) R1 += col_first;
+        const dim3 grid(tiles, D.K, nops), block(tc * T);
+        const size_t lds = col_lds_bytes(nb, (int)tc);
+        return inverse ? col_dispatch<true>(nb, grid, block, lds, s, D, R0, R1, (int)tc, 0, D.w2, tiles_log)
+                       : col_dispatch<false>(nb, grid, block, lds, s, D, R0, R1, (int)tc, 0, D.w2, tiles_log);
+    }
     // the strided passes 0 .. npass - 2 (forward in order, inverse in reverse)
     const int last = D.npass - 1;
     for (int k = 0; k < last; ++k) {
@@ -1378,9 +1425,23 @@ hipError_t launch_row(const DevPlan& D, const LaunchCfg& C, u32* RB, const u32* 
 }
 
 hipError_t launch_crt(const DevPlan& D, const LaunchCfg& C, const u32* R, u32* out, const uint2* G, hipStream_t s,
-                      u32 first, u32 count, long long* part) {
+                      u32 first, u32 count, long long* part, int lcw) {
     if (count == 0) count = D.N - first;
     if (first + count > D.N) return hipErrorInvalidValue;
+    if (lcw) {  // column windows: the aliased CRT only
+        if (!C.crt_alias || !window_ok(D, lcw, C.crt_tn, first, count)) return hipErrorInvalidValue;
+        const int koff = (int)crt_alias_kc_off(C.crt_tn, C.crt_ys, D.L);
+        const dim3 grid(count / C.crt_tn);
+        R += residue_first(D, lcw, first);
+        out += (size_t)first * D.L;
+        if (C.crt_tn == 64)
+            hipLaunchKernelGGL((k_crt_alias<2, 8>), grid, dim3(512), C.crt_lds, s, D, R, out, G, count, C.crt_ys, koff,
+                               lcw);
+        else
+            hipLaunchKernelGGL((k_crt_alias<1, 8>), grid, dim3(256), C.crt_lds, s, D, R, out, G, count, C.crt_ys, koff,
+                               lcw);
+        return hipGetLastError();
+    }
     const dim3 grid((count + C.crt_tn - 1) / C.crt_tn), block(256);
     if (C.crt_split) {
         // slots of [first, first + count) into part (stride N per slot), then the finish
@@ -1395,9 +1456,9 @@ hipError_t launch_crt(const DevPlan& D, const LaunchCfg& C, const u32* R, u32* o
     if (C.crt_alias) {
         const int koff = (int)crt_alias_kc_off(C.crt_tn, C.crt_ys, D.L);
         if (C.crt_tn == 64)
-            hipLaunchKernelGGL((k_crt_alias<2, 8>), grid, dim3(512), C.crt_lds, s, D, R, out, G, count, C.crt_ys, koff);
+            hipLaunchKernelGGL((k_crt_alias<2, 8>), grid, dim3(512), C.crt_lds, s, D, R, out, G, count, C.crt_ys, koff, 0);
         else
-            hipLaunchKernelGGL((k_crt_alias<1, 8>), grid, block, C.crt_lds, s, D, R, out, G, count, C.crt_ys, koff);
+            hipLaunchKernelGGL((k_crt_alias<1, 8>), grid, block, C.crt_lds, s, D, R, out, G, count, C.crt_ys, koff, 0);
         return hipGetLastError();
     }
     switch (C.crt_tn) {

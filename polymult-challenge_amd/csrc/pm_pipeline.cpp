@@ -18,10 +18,14 @@
 //     the streams, and the host team scatters each chunk from pinned staging
 //     into the caller's buffer while the next is copied.
 // Chunks are contiguous coefficient ranges (the input layout is [N][L]), so
-// staging needs no gather; the column NTT pass simply starts after the last
-// input chunk instead of per chunk. Points whose plans are single chunks (a at
-// most 1 MiB) skip the copy engines: the kernels read and write the pinned
-// staging buffers directly.
+// staging needs no gather and the column NTT pass starts after the last input
+// chunk. Column windows are the other option (PM_IN_COLS, PM_OUT_COLS; off by
+// default, measured no faster): b staged one window of columns at a time (each
+// row's run of CW coefficients), each window's RNS and forward column pass
+// running as it lands; the result leaving first window first, the other
+// columns' inverse column pass under its copy. Points whose plans are single
+// chunks (a at most 1 MiB) skip the copy engines: the kernels read and write
+// the pinned staging buffers directly.
 //
 // PM_PROFILE=1 prints one phase line per call on stderr (the chunked pipeline).
 #include <hip/hip_runtime.h>
@@ -115,6 +119,11 @@ struct Pipeline {
         uint32_t first, count;
     };
     std::vector<Chunk> cin[2], cout;  // input plans of a and b, output plan
+    // Column windows of 2^lcw columns (0: contiguous coefficient chunks): b
+    // staged window by window (cin[1] then holds staging positions, one window
+    // per chunk), and the result leaving first window first (cout likewise).
+    int lcw_in = 0, lcw_out = 0;
+    LaunchCfg cfg_in, cfg_out;  // the RNS and CRT tiles sized to fit a window
     std::vector<hipEvent_t> ev_in, ev_crt, ev_out;
     bool profile = false;
     // PM_PROFILE timeline: start of input, last H2D done, last RNS done,
@@ -227,6 +236,58 @@ void State::pipeline_init() {
                             sizes_from_env("PM_IN_TAIL_B", {}), 128);
     P->cout = plan_chunks(host.N, bpc, sizes_from_env("PM_OUT_CHUNKS", {MiB, 5 * MiB}),
                           sizes_from_env("PM_OUT_TAIL", {MiB / 2}), 128);
+    // Column windows, for chunked two-pass plans only (PM_IN_COLS and
+    // PM_OUT_COLS: window width in columns, 0 for contiguous chunks).
+    const bool chunked = !(P->cin[0].size() == 1 && P->cin[1].size() == 1 && P->cout.size() == 1);
+    auto window_log = [&](const char* name, int dflt) {
+        const char* e = std::getenv(name);
+        const int cols = e ? std::atoi(e) : dflt;
+        if (!chunked || dev.npass != 2 || cols < 32 || (cols & (cols - 1)) || cols % cfg.col_tc ||
+            cols >= (1 << host.w2))
+            return 0;
+        int l = 0;
+        while ((1 << l) < cols) ++l;
+        return l;
+    };
+    const uint32_t rows = 1u << host.w1;
+    P->lcw_in = window_log("PM_IN_COLS", 0);
+    if (P->lcw_in) {
+        P->cfg_in = cfg;
+        if (P->cfg_in.rns_tn > (1 << P->lcw_in)) {
+            P->cfg_in.rns_tn = 1 << P->lcw_in;
+            P->cfg_in.rns_lds = rns_lds_bytes(P->cfg_in.rns_tn, (int)host.L, dev.rns_rt);
+        }
+        const uint32_t win = rows << P->lcw_in;
+        P->cin[1].clear();
+        for (uint32_t f = 0; f < host.N; f += win) P->cin[1].push_back({f, win});
+    }
+    // Both are off by default. At 65536/1747, per call against contiguous
+    // chunks (alternating rounds of 100 calls): 64-column output windows
+    // 0.984-0.999 ms against 0.990-1.031 on one box, 1.000-1.009 against
+    // 1.000-1.008 on another; b in 128-column windows as well 0.987-1.009 ms;
+    // b in 64-column windows 1.005-1.008 against 0.987-0.997. The windows
+    // take the inverse column pass off the first output chunk's path, but the
+    // row pass before it cannot be windowed, the first window is a larger first
+    // copy, and b's windows are more and smaller copies.
+    P->lcw_out = cfg.crt_alias ? window_log("PM_OUT_COLS", 0) : 0;
+    if (P->lcw_out) {
+        P->cfg_out = cfg;
+        if (P->cfg_out.crt_tn > (1 << P->lcw_out)) {
+            P->cfg_out.crt_tn = 1 << P->lcw_out;
+            P->cfg_out.crt_lds = crt_alias_lds_bytes(P->cfg_out.crt_tn, cfg.crt_ys, (int)host.L, dev.crt_ks);
+        }
+        // the first window alone, then the usual chunks over the rest, each a
+        // whole number of window rows (runs of 2^lcw coefficients)
+        const uint32_t win = rows << P->lcw_out;
+        // (4 MiB chunks and a 1 + 0.5 MiB tail, so the last scatter is short:
+        // 5 MiB chunks and a 0.5 MiB tail left it exposed, 1.003-1.019 ms)
+        std::vector<size_t> sz = sizes_from_env("PM_OUT_CHUNKS", {MiB, 4 * MiB});
+        if (sz.size() > 1) sz.erase(sz.begin());
+        P->cout = {{0, win}};
+        for (const auto ch : plan_chunks(host.N - win, bpc, sz, sizes_from_env("PM_OUT_TAIL", {MiB, MiB / 2}),
+                                         1u << P->lcw_out))
+            P->cout.push_back({win + ch.first, ch.count});
+    }
     if (P->profile) {
         for (int i = 0; i < 3; ++i) hip_check(hipEventCreate(&P->tl[i]), "event");
         for (auto& e : P->tl_more) hip_check(hipEventCreate(&e), "event");
@@ -239,9 +300,11 @@ void State::pipeline_init() {
     mk(P->ev_crt, P->cout.size());
     mk(P->ev_out, P->cout.size());
     if (P->profile)
-        std::fprintf(stderr, "pm: N=%u W=%u L=%u K=%d, w1=%d w2=%d, chunks in a %zu / b %zu / out %zu, %d host threads\n",
+        std::fprintf(stderr,
+                     "pm: N=%u W=%u L=%u K=%d, w1=%d w2=%d, chunks in a %zu / b %zu / out %zu, column windows in %d / "
+                     "out %d, %d host threads\n",
                      host.N, host.W, host.L, host.K, host.w1, host.w2, P->cin[0].size(), P->cin[1].size(), P->cout.size(),
-                     P->team->size());
+                     P->lcw_in ? 1 << P->lcw_in : 0, P->lcw_out ? 1 << P->lcw_out : 0, P->team->size());
 }
 
 void State::pipeline_free() {
@@ -292,6 +355,25 @@ void State::run_host(const uint32_t* a, const uint32_t* b, uint32_t* c) {
         });
     };
 
+    // Staging positions [f, f + cnt) of a column-window order (runs of 2^lcw
+    // coefficients; run u is row u mod 2^w1 of window u / 2^w1) to or from
+    // their places in an [N][L] array, the runs split over the team.
+    auto move_runs = [&](uint32_t* to, const uint32_t* from, uint32_t f, uint32_t cnt, int lcw, bool gather) {
+        const uint32_t u0 = f >> lcw, nu = cnt >> lcw, rmask = (1u << host.w1) - 1u;
+        const size_t run = L << lcw;
+        const uint32_t per = (nu + T - 1) / T;
+        P.team->run(T, [&](int t) {
+            for (uint32_t u = u0 + (uint32_t)t * per, e = std::min(u0 + nu, u + per); u < e; ++u) {
+                const size_t pos = (size_t)u * run;
+                const size_t n = (((size_t)(u & rmask) << host.w2) + ((size_t)(u >> host.w1) << lcw)) * L;
+                if (gather)
+                    copy_stream(to + pos, from + n, run * 4);
+                else
+                    copy_stream(to + n, from + pos, run * 4);
+            }
+        });
+    };
+
     // ---- small points: one chunk each way, nothing to overlap ----
     // Every plan is a single chunk (a at most 1 MiB), so the pipeline's copy
     // streams and cross-stream events would only add their latency. Both
@@ -334,7 +416,10 @@ void State::run_host(const uint32_t* a, const uint32_t* b, uint32_t* c) {
         for (const auto ch : P.cin[op]) {
             ++i;
             const size_t off = (size_t)ch.first * L, words = (size_t)ch.count * L;
-            copy_job(P.h_in[op] + off, src[op] + off, words);
+            if (op == 1 && P.lcw_in)
+                move_runs(P.h_in[1], b, ch.first, ch.count, P.lcw_in, true);
+            else
+                copy_job(P.h_in[op] + off, src[op] + off, words);
             hipStream_t sc = (P.s_in2 && (i & 1)) ? P.s_in2 : P.s_in;
             hip_check(hipMemcpyAsync(d_in[op] + off, P.h_in[op] + off, words * 4, hipMemcpyHostToDevice, sc), "H2D");
             hip_check(hipEventRecord(P.ev_in[i], sc), "record");
@@ -347,6 +432,15 @@ void State::run_host(const uint32_t* a, const uint32_t* b, uint32_t* c) {
         for (const auto ch : P.cin[op]) {
             ++i;
             hip_check(hipStreamWaitEvent(stream, P.ev_in[i], 0), "wait");
+            if (op == 1 && P.lcw_in) {  // b's window: its RNS, then its columns' forward pass
+                hip_check(launch_rns(dev, P.cfg_in, d_in[1], nullptr, d_R[1], nullptr, 1, stream, ch.first, ch.count,
+                                     P.lcw_in),
+                          "rns");
+                hip_check(launch_col(dev, cfg, d_R[1], nullptr, 1, false, stream, ch.first >> host.w1,
+                                     ch.count >> host.w1),
+                          "col b");
+                continue;
+            }
             hip_check(launch_rns(dev, cfg, d_in[op], nullptr, d_R[op], nullptr, 1, stream, ch.first, ch.count), "rns");
         }
         // a's column pass and forward rows run while b is still crossing the link
@@ -362,22 +456,31 @@ void State::run_host(const uint32_t* a, const uint32_t* b, uint32_t* c) {
     }
 
     // ---- the rest of the NTT passes ----
-    hip_check(launch_col(dev, cfg, d_R[1], nullptr, 1, false, stream), "col b");
+    if (!P.lcw_in) hip_check(launch_col(dev, cfg, d_R[1], nullptr, 1, false, stream), "col b");
     hip_check(launch_row(dev, cfg, d_R[1], d_R[0], ROW_FUSED_PRE, stream), "row");
-    hip_check(launch_col(dev, cfg, d_R[1], nullptr, 1, true, stream), "col inv");
+    const int nout = (int)P.cout.size();
+    const uint32_t first_n = P.cout[0].count;
+    const int lo = P.lcw_out;
+    // with output windows, the first window's columns only (the rest follow
+    // its CRT, under its copy)
+    hip_check(lo ? launch_col(dev, cfg, d_R[1], nullptr, 1, true, stream, 0, 1u << lo)
+                 : launch_col(dev, cfg, d_R[1], nullptr, 1, true, stream),
+              "col inv");
     if (P.profile) hip_check(hipEventRecord(P.tl_more[0], stream), "record");
 
     // ---- output: CRT, then D2H per chunk ----
     // The CRT of the first chunk runs alone so its copy starts early; the rest
     // runs as one full-width launch (small CRT launches leave most CUs idle and
     // would pace the copies). Each D2H waits for the launch that covers it.
-    const int nout = (int)P.cout.size();
-    const uint32_t first_n = P.cout[0].count;
-    hip_check(launch_crt(dev, cfg, d_R[1], d_out, dev.G, stream, 0, first_n, d_slots), "crt");
+    const LaunchCfg& co = lo ? P.cfg_out : cfg;
+    hip_check(launch_crt(dev, co, d_R[1], d_out, dev.G, stream, 0, first_n, d_slots, lo), "crt");
     hip_check(hipEventRecord(P.ev_crt[0], stream), "record");
     if (P.profile) hip_check(hipEventRecord(P.tl_more[1], stream), "record");
     if (nout > 1) {
-        hip_check(launch_crt(dev, cfg, d_R[1], d_out, dev.G, stream, first_n, host.N - first_n, d_slots), "crt");
+        if (lo)
+            hip_check(launch_col(dev, cfg, d_R[1], nullptr, 1, true, stream, 1u << lo, (1u << host.w2) - (1u << lo)),
+                      "col inv");
+        hip_check(launch_crt(dev, co, d_R[1], d_out, dev.G, stream, first_n, host.N - first_n, d_slots, lo), "crt");
         hip_check(hipEventRecord(P.ev_crt[1], stream), "record");
     }
     release_R(stream);
@@ -400,7 +503,10 @@ void State::run_host(const uint32_t* a, const uint32_t* b, uint32_t* c) {
         if (k == 0) t_first = clock::now();
         const auto ch = P.cout[k];
         const size_t off = (size_t)ch.first * L;
-        copy_job(c + off, P.h_out + off, (size_t)ch.count * L);
+        if (lo)
+            move_runs(c, P.h_out, ch.first, ch.count, lo, false);
+        else
+            copy_job(c + off, P.h_out + off, (size_t)ch.count * L);
     }
     if (P.profile) {
         auto us = [&](clock::time_point t) { return std::chrono::duration<double, std::micro>(t - t0).count(); };

@@ -438,3 +438,51 @@ def test_host_pipeline_plans(O, pm, env, monkeypatch):
                 a, b = O.operands(kind, N, W, q, 1300 + s + N)
                 got = m.run(a, b)
                 assert np.array_equal(got, P.multiply(a, b)), (N, W, kind)
+
+
+@pytest.mark.parametrize("env", [
+    {"PM_IN_COLS": "32", "PM_OUT_COLS": "32"},
+    {"PM_IN_COLS": "64", "PM_OUT_COLS": "64"},
+    {"PM_IN_COLS": "128", "PM_OUT_COLS": "128", "PM_OUT_CHUNKS": "0.3,2"},
+    {"PM_IN_COLS": "64"},
+    {"PM_OUT_COLS": "128", "PM_HOST_THREADS": "3"},
+    {"PM_OUT_COLS": "0"},
+])
+def test_host_column_windows(O, pm, env, monkeypatch):
+    """pm_run with b staged in column windows (each window's RNS and forward
+    column pass run as it lands) and the result leaving window by window (the
+    first window's inverse column pass and CRT first), against the oracle.
+    Windows apply to chunked two-pass plans: 65536/1747 (w1 = w2 = 8),
+    32768/868 (w1 = 7, w2 = 8), 16384/288 (128 columns: a 128-column window
+    is the whole row and stays off), 16384/5000 (64-bit RNS epilogue; its
+    CRT is not the aliased one, so the result stays in contiguous chunks) and
+    131072/64 (three passes: no windows). PM_OUT_COLS=0 is the default,
+    contiguous chunks."""
+    for k, v in env.items():
+        monkeypatch.setenv(k, v)
+    print("column windows %s" % env, flush=True)
+    for N, W, kinds in [(65536, 1747, ["mixed"]), (32768, 868, ["mixed", "max"]), (16384, 288, ["uniform"]),
+                        (16384, 5000, ["mixed"]), (131072, 64, ["max"])]:
+        q, L, m, P = _setup(O, pm, N, W)
+        with m:
+            for s, kind in enumerate(kinds):
+                a, b = O.operands(kind, N, W, q, 1900 + s + N)
+                got = m.run(a, b)
+                assert np.array_equal(got, P.multiply(a, b)), (N, W, kind, env)
+
+
+def test_host_column_windows_engage(O, pm, monkeypatch, capfd):
+    """The window settings reach the pipeline: with PM_PROFILE its set-up
+    line names the window widths in use at 65536/1747, and the profiled call
+    is exact too."""
+    monkeypatch.setenv("PM_IN_COLS", "64")
+    monkeypatch.setenv("PM_OUT_COLS", "32")
+    monkeypatch.setenv("PM_PROFILE", "1")
+    N, W = 65536, 1747
+    q, L, m, P = _setup(O, pm, N, W)
+    with m:
+        a, b = O.operands("mixed", N, W, q, 1999)
+        got = m.run(a, b)
+        assert np.array_equal(got, P.multiply(a, b))
+    err = capfd.readouterr().err
+    assert "column windows in 64 / out 32" in err, err[-2000:]

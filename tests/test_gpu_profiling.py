@@ -49,7 +49,8 @@ def test_pm_profile_timeline(O, pm, monkeypatch, capfd):
     print(err, flush=True)
     want = O.OraclePlan(N, W, q).multiply(a, b)
     assert all(np.array_equal(g, want) for g in got)
-    assert re.search(r"pm: N=8192 W=438 L=14 K=\d+, w1=6 w2=7, chunks in a \d+ / b \d+ / out \d+, \d+ host threads", err)
+    assert re.search(r"pm: N=8192 W=438 L=14 K=\d+, w1=6 w2=7, chunks in a \d+ / b \d+ / out \d+, "
+                     r"column windows in 0 / out 0, \d+ host threads", err)
     host = re.findall(r"pm: inputs staged\+submitted ([\d.]+) us \| first out chunk ([\d.]+) \| done ([\d.]+)", err)
     gpu = re.findall(r"pm: gpu: last H2D ([\d.]+) us \| last RNS ([\d.]+) \| NTT passes ([\d.]+) \| first CRT ([\d.]+) "
                      r"\| first D2H ([\d.]+) \| last D2H ([\d.]+)", err)
