@@ -484,11 +484,21 @@ void State::run_host(const uint32_t* a, const uint32_t* b, uint32_t* c) {
         hip_check(hipEventRecord(P.ev_crt[1], stream), "record");
     }
     release_R(stream);
+    // Chunk 0 waits for the first CRT launch; each D2H stream's first chunk
+    // after it waits for the second (which follows the first on the compute
+    // stream), and the stream's later chunks follow that copy.
+    bool waited_rest[2] = {false, false};
     for (int k = 0; k < nout; ++k) {
         const auto ch = P.cout[k];
         const size_t off = (size_t)ch.first * L, words = (size_t)ch.count * L;
-        hipStream_t so = (P.s_out2 && (k & 1)) ? P.s_out2 : P.s_out;
-        if (k < 2 || so != P.s_out) hip_check(hipStreamWaitEvent(so, P.ev_crt[std::min(k, 1)], 0), "wait");
+        const int si = (P.s_out2 && (k & 1)) ? 1 : 0;
+        hipStream_t so = si ? P.s_out2 : P.s_out;
+        if (k == 0) {
+            hip_check(hipStreamWaitEvent(so, P.ev_crt[0], 0), "wait");
+        } else if (!waited_rest[si]) {
+            hip_check(hipStreamWaitEvent(so, P.ev_crt[1], 0), "wait");
+            waited_rest[si] = true;
+        }
         hip_check(hipMemcpyAsync(P.h_out + off, d_out + off, words * 4, hipMemcpyDeviceToHost, so), "D2H");
         hip_check(hipEventRecord(P.ev_out[k], so), "record");
         if (P.profile && k == 0) hip_check(hipEventRecord(P.tl_more[2], so), "record");

@@ -486,3 +486,24 @@ def test_host_column_windows_engage(O, pm, monkeypatch, capfd):
         assert np.array_equal(got, P.multiply(a, b))
     err = capfd.readouterr().err
     assert "column windows in 64 / out 32" in err, err[-2000:]
+
+
+def test_host_output_chunks_wait_for_their_crt(O, pm, monkeypatch):
+    """Every D2H chunk waits for the CRT launch that writes it. With a
+    128-coefficient first chunk its copy ends long before the second CRT
+    launch (the rest of the coefficients) does, so a copy stream that waited
+    only for the first launch could copy stale results of the previous call;
+    each call here has new operands, so stale chunks would not match. (On
+    this runtime the two D2H streams' copies went out in issue order even
+    without that wait, so the check guards the dependency rather than
+    reproducing its absence.)"""
+    monkeypatch.setenv("PM_OUT_CHUNKS", "0.02,4")
+    monkeypatch.setenv("PM_OUT_TAIL", "0")
+    N, W = 65536, 1747
+    q, L, m, P = _setup(O, pm, N, W)
+    with m:
+        for s, kind in enumerate(["mixed", "uniform", "max", "mixed"]):
+            a, b = O.operands(kind, N, W, q, 2100 + s)
+            got = m.run(a, b)
+            bad = _diff(got, P.multiply(a, b))
+            assert bad == 0, (kind, s, bad)
