@@ -249,6 +249,14 @@ void State::pipeline_init() {
         while ((1 << l) < cols) ++l;
         return l;
     };
+    // Both are off by default. At 65536/1747, per call against contiguous
+    // chunks (alternating rounds of 100 calls): 64-column output windows
+    // 0.984-0.999 ms against 0.990-1.031 on one box, 1.000-1.009 against
+    // 1.000-1.008 on another; b in 128-column windows as well 0.987-1.009 ms;
+    // b in 64-column windows 1.005-1.008 against 0.987-0.997. The windows
+    // take the inverse column pass off the first output chunk's path, but the
+    // row pass before it cannot be windowed, the first window is a larger first
+    // copy, and b's windows are more and smaller copies.
     const uint32_t rows = 1u << host.w1;
     P->lcw_in = window_log("PM_IN_COLS", 0);
     if (P->lcw_in) {
@@ -261,14 +269,6 @@ void State::pipeline_init() {
         P->cin[1].clear();
         for (uint32_t f = 0; f < host.N; f += win) P->cin[1].push_back({f, win});
     }
-    // Both are off by default. At 65536/1747, per call against contiguous
-    // chunks (alternating rounds of 100 calls): 64-column output windows
-    // 0.984-0.999 ms against 0.990-1.031 on one box, 1.000-1.009 against
-    // 1.000-1.008 on another; b in 128-column windows as well 0.987-1.009 ms;
-    // b in 64-column windows 1.005-1.008 against 0.987-0.997. The windows
-    // take the inverse column pass off the first output chunk's path, but the
-    // row pass before it cannot be windowed, the first window is a larger first
-    // copy, and b's windows are more and smaller copies.
     P->lcw_out = cfg.crt_alias ? window_log("PM_OUT_COLS", 0) : 0;
     if (P->lcw_out) {
         P->cfg_out = cfg;
