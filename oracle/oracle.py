@@ -19,6 +19,7 @@ What is computed here, and what pins it:
 """
 from __future__ import annotations
 
+import contextlib
 import ctypes
 import json
 import os
@@ -289,29 +290,46 @@ def read_results(root) -> dict:
 _LIB = None
 
 
+def _bind(path):
+    L = ctypes.CDLL(str(path))
+    u32p = ctypes.POINTER(ctypes.c_uint32)
+    L.pmo_plan_create.argtypes = [ctypes.c_uint32, ctypes.c_uint32, ctypes.c_uint32, u32p,
+                                  ctypes.POINTER(ctypes.c_void_p), ctypes.c_char_p, ctypes.c_size_t]
+    L.pmo_plan_create.restype = ctypes.c_int
+    L.pmo_plan_free.argtypes = [ctypes.c_void_p]
+    L.pmo_plan_info.argtypes = [ctypes.c_void_p, u32p, ctypes.POINTER(ctypes.c_uint64), u32p, u32p]
+    for name in ("pmo_rns", "pmo_crt"):
+        getattr(L, name).argtypes = [ctypes.c_void_p, u32p, u32p]
+    L.pmo_crt.restype = ctypes.c_int
+    for name in ("pmo_ntt_fwd", "pmo_ntt_inv"):
+        getattr(L, name).argtypes = [ctypes.c_void_p, u32p]
+    L.pmo_pointwise.argtypes = [ctypes.c_void_p, u32p, u32p, u32p]
+    L.pmo_multiply.argtypes = [ctypes.c_void_p, u32p, u32p, u32p]
+    L.pmo_multiply.restype = ctypes.c_int
+    return L
+
+
 def lib():
     global _LIB
     if _LIB is None:
         path = HERE / "liboracle.so"
         if not path.exists():
             raise RuntimeError("oracle/liboracle.so is not built (make -C oracle)")
-        L = ctypes.CDLL(str(path))
-        u32p = ctypes.POINTER(ctypes.c_uint32)
-        L.pmo_plan_create.argtypes = [ctypes.c_uint32, ctypes.c_uint32, ctypes.c_uint32, u32p,
-                                      ctypes.POINTER(ctypes.c_void_p), ctypes.c_char_p, ctypes.c_size_t]
-        L.pmo_plan_create.restype = ctypes.c_int
-        L.pmo_plan_free.argtypes = [ctypes.c_void_p]
-        L.pmo_plan_info.argtypes = [ctypes.c_void_p, u32p, ctypes.POINTER(ctypes.c_uint64), u32p, u32p]
-        for name in ("pmo_rns", "pmo_crt"):
-            getattr(L, name).argtypes = [ctypes.c_void_p, u32p, u32p]
-        L.pmo_crt.restype = ctypes.c_int
-        for name in ("pmo_ntt_fwd", "pmo_ntt_inv"):
-            getattr(L, name).argtypes = [ctypes.c_void_p, u32p]
-        L.pmo_pointwise.argtypes = [ctypes.c_void_p, u32p, u32p, u32p]
-        L.pmo_multiply.argtypes = [ctypes.c_void_p, u32p, u32p, u32p]
-        L.pmo_multiply.restype = ctypes.c_int
-        _LIB = L
+        _LIB = _bind(path)
     return _LIB
+
+
+@contextlib.contextmanager
+def library(path):
+    """Plans created inside use another build of pm_oracle.c at `path` (the
+    tests' planted-mistake builds); each plan keeps the build it was made with."""
+    global _LIB
+    saved = lib()
+    _LIB = _bind(path)
+    try:
+        yield
+    finally:
+        _LIB = saved
 
 
 def _ptr(a: np.ndarray):
@@ -329,51 +347,52 @@ class OraclePlan:
         self.N, self.W, self.L = N, W, L
         h = ctypes.c_void_p()
         err = ctypes.create_string_buffer(256)
-        rc = lib().pmo_plan_create(N, W, L, _ptr(ql), ctypes.byref(h), err, 256)
+        self._lib = lib()
+        rc = self._lib.pmo_plan_create(N, W, L, _ptr(ql), ctypes.byref(h), err, 256)
         if rc != 0:
             raise RuntimeError(err.value.decode())
         self._h = h
         K = ctypes.c_uint32()
         delta = ctypes.c_uint64()
-        lib().pmo_plan_info(h, ctypes.byref(K), ctypes.byref(delta), None, None)
+        self._lib.pmo_plan_info(h, ctypes.byref(K), ctypes.byref(delta), None, None)
         self.K, self.delta = K.value, delta.value
         self.primes = np.zeros(self.K, dtype=np.uint32)
         self.psi = np.zeros(self.K, dtype=np.uint32)
-        lib().pmo_plan_info(h, None, None, _ptr(self.primes), _ptr(self.psi))
+        self._lib.pmo_plan_info(h, None, None, _ptr(self.primes), _ptr(self.psi))
 
     def __del__(self):
         h = getattr(self, "_h", None)
-        if h is not None and _LIB is not None:
-            _LIB.pmo_plan_free(h)
+        if h is not None:
+            self._lib.pmo_plan_free(h)
             self._h = None
 
     def rns(self, a: np.ndarray) -> np.ndarray:
         a = np.ascontiguousarray(a, dtype=np.uint32)
         R = np.zeros((self.K, self.N), dtype=np.uint32)
-        lib().pmo_rns(self._h, _ptr(a), _ptr(R))
+        self._lib.pmo_rns(self._h, _ptr(a), _ptr(R))
         return R
 
     def ntt_fwd(self, R: np.ndarray) -> np.ndarray:
         R = np.array(R, dtype=np.uint32, order="C", copy=True)
-        lib().pmo_ntt_fwd(self._h, _ptr(R))
+        self._lib.pmo_ntt_fwd(self._h, _ptr(R))
         return R
 
     def ntt_inv(self, R: np.ndarray) -> np.ndarray:
         R = np.array(R, dtype=np.uint32, order="C", copy=True)
-        lib().pmo_ntt_inv(self._h, _ptr(R))
+        self._lib.pmo_ntt_inv(self._h, _ptr(R))
         return R
 
     def pointwise(self, A: np.ndarray, B: np.ndarray) -> np.ndarray:
         A = np.ascontiguousarray(A, dtype=np.uint32)
         B = np.ascontiguousarray(B, dtype=np.uint32)
         C = np.zeros_like(A)
-        lib().pmo_pointwise(self._h, _ptr(A), _ptr(B), _ptr(C))
+        self._lib.pmo_pointwise(self._h, _ptr(A), _ptr(B), _ptr(C))
         return C
 
     def crt(self, X: np.ndarray) -> np.ndarray:
         X = np.ascontiguousarray(X, dtype=np.uint32)
         c = np.zeros((self.N, self.L), dtype=np.uint32)
-        if lib().pmo_crt(self._h, _ptr(X), _ptr(c)):
+        if self._lib.pmo_crt(self._h, _ptr(X), _ptr(c)):
             raise ValueError("CRT margin violated: operands outside [0, q)")
         return c
 
@@ -381,7 +400,7 @@ class OraclePlan:
         a = np.ascontiguousarray(a, dtype=np.uint32)
         b = np.ascontiguousarray(b, dtype=np.uint32)
         c = np.zeros((self.N, self.L), dtype=np.uint32)
-        if lib().pmo_multiply(self._h, _ptr(a), _ptr(b), _ptr(c)):
+        if self._lib.pmo_multiply(self._h, _ptr(a), _ptr(b), _ptr(c)):
             raise ValueError("CRT margin violated: operands outside [0, q)")
         return c
 

@@ -21,6 +21,20 @@ typedef uint32_t u32;
 typedef uint64_t u64;
 typedef unsigned __int128 u128;
 
+/* PMO_MUTATION (0 by default: none) plants one deliberate mistake, so that
+ * tests/test_oracle.py can show that the independent checks pinning this file
+ * catch such slips:
+ *   1 one prime fewer than the K rule
+ *   2 the inverse butterfly's difference taken with the wrong sign
+ *   3 the T[v] = (-v M) mod q term dropped from the CRT
+ *   4 the N^-1 scaling dropped from the inverse NTT
+ *   5 the final conditional subtraction of q dropped
+ *   6 a's transform squared instead of multiplied by b's
+ *   7 the fold adding z >> W without the factor delta */
+#ifndef PMO_MUTATION
+#define PMO_MUTATION 0
+#endif
+
 struct pmo_plan {
     u32 N, W, L;
     int logN, K;
@@ -198,6 +212,9 @@ int pmo_plan_create(u32 N, u32 W, u32 L, const u32* q, pmo_plan** out, char* err
         return -1;
     }
 
+#if PMO_MUTATION == 1
+    if (K > 1) --K;
+#endif
     pmo_plan* P = (pmo_plan*)calloc(1, sizeof(pmo_plan));
     P->N = N; P->W = W; P->L = L; P->logN = logN; P->K = K; P->delta = delta;
     P->q = (u32*)malloc((size_t)L * 4);
@@ -364,13 +381,18 @@ void pmo_ntt_inv(const pmo_plan* P, u32* R) {
                 for (u32 k = j1; k < j1 + t; ++k) {
                     const u32 U = x[k], V = x[k + t];
                     x[k] = U + V >= p ? U + V - p : U + V;
+#if PMO_MUTATION == 2
+                    x[k + t] = (u32)((u64)(V >= U ? V - U : V + p - U) * w % p);
+#else
                     x[k + t] = (u32)((u64)(U >= V ? U - V : U + p - V) * w % p);
+#endif
                 }
             }
             t <<= 1;
         }
         const u64 ni = P->ninv[j];
-        for (u32 k = 0; k < N; ++k) x[k] = (u32)(x[k] * ni % p);
+        if (PMO_MUTATION != 4)
+            for (u32 k = 0; k < N; ++k) x[k] = (u32)(x[k] * ni % p);
     }
 }
 
@@ -380,7 +402,7 @@ void pmo_pointwise(const pmo_plan* P, const u32* A, const u32* B, u32* C) {
         const u64 p = P->p[j];
         for (u32 n = 0; n < N; ++n) {
             const size_t i = (size_t)j * N + n;
-            C[i] = (u32)((u64)A[i] * B[i] % p);
+            C[i] = (u32)((u64)A[i] * (PMO_MUTATION == 6 ? A[i] : B[i]) % p);
         }
     }
 }
@@ -423,7 +445,7 @@ int pmo_crt(const pmo_plan* P, const u32* X, u32* c) {
                 carry = s >> 32;
             }
         }
-        {
+        if (PMO_MUTATION != 3) {
             const u32* Tv = P->T + (size_t)v * L;
             u64 carry = 0;
             for (int l = 0; l < ZL; ++l) {
@@ -441,7 +463,7 @@ int pmo_crt(const pmo_plan* P, const u32* X, u32* c) {
             if (hi == 0) break;
             z[wl] = wb ? (z[wl] & ((1u << wb) - 1u)) : 0u;
             for (int l = wl + 1; l < ZL; ++l) z[l] = 0;
-            const u128 add = hi * (u128)P->delta;
+            const u128 add = hi * (u128)(PMO_MUTATION == 7 ? 1u : P->delta);
             u128 carry = 0;
             for (int l = 0; l < ZL; ++l) {
                 const u128 s = (u128)z[l] + (l < 4 ? (u32)(add >> (32 * l)) : 0u) + carry;
@@ -449,7 +471,7 @@ int pmo_crt(const pmo_plan* P, const u32* X, u32* c) {
                 carry = s >> 32;
             }
         }
-        if (big_cmp(z, P->q, (int)L) >= 0) big_sub(z, P->q, (int)L);
+        if (PMO_MUTATION != 5 && big_cmp(z, P->q, (int)L) >= 0) big_sub(z, P->q, (int)L);
         memcpy(c + (size_t)n * L, z, (size_t)L * 4);
     }
     free(z);

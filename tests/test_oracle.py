@@ -186,3 +186,72 @@ def test_plan_errors_match_reference(O):
         O.OraclePlan(64, 32, np.array([q - 1], np.uint32))
     with pytest.raises(RuntimeError, match="too far below"):
         O.OraclePlan(64, 32, np.array([3], np.uint32))
+
+
+# Planted mistakes (oracle/pm_oracle.c, PMO_MUTATION): each is a slip the checks
+# above are chosen to catch. The reference's README claims its tests catch two
+# such bugs (too few primes, a wrong inverse-butterfly sign) with no harness to
+# show it; here each planted build must fail at least one end-to-end check.
+MUTANTS = {
+    1: "one prime fewer than the K rule",
+    2: "inverse butterfly difference with the wrong sign",
+    3: "T[v] term dropped from the CRT",
+    4: "N^-1 dropped from the inverse NTT",
+    5: "final conditional subtraction of q dropped",
+    6: "a's transform squared instead of multiplied by b's",
+    7: "fold without the factor delta",
+}
+ORACLE_C = Path(__file__).resolve().parent.parent / "oracle" / "pm_oracle.c"
+
+
+def _oracle_build(tmp_path, m):
+    out = tmp_path / ("liboracle_m%d.so" % m)
+    subprocess.run(["gcc", "-O2", "-std=c11", "-fPIC", "-shared", "-DPMO_MUTATION=%d" % m, "-o", str(out),
+                    str(ORACLE_C), "-lm"], check=True)
+    return out
+
+
+def _end_to_end_failures(O):
+    """The end-to-end checks of this file (schoolbook, Kronecker, the
+    all-(q-1) closed form) that the oracle build in use fails."""
+    failed = []
+
+    def check(name, fn):
+        try:
+            ok = fn()
+        except ValueError:  # the CRT's margin check
+            ok = False
+        if not ok:
+            failed.append(name)
+
+    for N, W in [(1, 8), (2, 16), (4, 20), (8, 33), (16, 64), (64, 32), (128, 256)]:
+        q = O.spec_q(N, W)
+
+        def school(N=N, W=W, q=q):
+            P = O.OraclePlan(N, W, q)
+            for s, kind in enumerate(["uniform", "max", "sparse", "small16", "mixed"]):
+                a, b = O.operands(kind, N, W, q, 5 + s)
+                if O.unpack(P.multiply(a, b)) != O.schoolbook(O.unpack(a), O.unpack(b), q):
+                    return False
+            return True
+
+        check("schoolbook %d/%d" % (N, W), school)
+    N, W = 1024, 512
+    q = O.spec_q(N, W)
+    a, b = O.operands("mixed", N, W, q, N + W)
+    check("kronecker", lambda: O.unpack(O.OraclePlan(N, W, q).multiply(a, b)) == O.kronecker(O.unpack(a), O.unpack(b), q))
+    N, W = 4096, 256
+    q = O.spec_q(N, W)
+    am = np.tile(O.q_limbs(q - 1, O.limbs_of(W)), (N, 1))
+    check("all q-1", lambda: O.unpack(O.OraclePlan(N, W, q).multiply(am, am)) == O.all_qm1(N, q))
+    return failed
+
+
+def test_planted_mistakes_are_caught(O, tmp_path):
+    with O.library(_oracle_build(tmp_path, 0)):
+        assert _end_to_end_failures(O) == []
+    for m, what in MUTANTS.items():
+        with O.library(_oracle_build(tmp_path, m)):
+            failed = _end_to_end_failures(O)
+        print("planted mistake %d (%s): caught by %s" % (m, what, ", ".join(failed) or "nothing"))
+        assert failed, "planted mistake %d (%s) passed every check" % (m, what)
