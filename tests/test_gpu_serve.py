@@ -84,7 +84,7 @@ def test_serve_admission_under_tiny_budget(O, pm):
 def test_serve_poisson_drive_short(O, pm):
     """Two seconds of open-loop Poisson load on the serving mix: every request
     completes, none fails, every result equals the plain path's product of
-    its operands (checked by the driver), percentiles are ordered."""
+    its operands (checked by the load driver, pm_server_drive), percentiles are ordered."""
     cat = _catalog(O, MIX_POINTS[:3])
     with pm.Server(cat) as S:
         rep = S.drive(500.0, 2.0, [0.70, 0.25, 0.05], seed=7)
@@ -112,9 +112,9 @@ def test_serve_saturate_closed_loop(O, pm):
 
 
 def test_serve_drive_expected_products_match_oracle(O, pm):
-    """The driver's reference for each result is the plain path (pm_run) at the
+    """The load driver's reference for each result is the plain path (pm_run) at the
     serving points; pin that path against the oracle at those points, on
-    operands uniform in [0, q) as the driver draws them."""
+    operands uniform in [0, q) as the load driver draws them."""
     for N, W in MIX_POINTS[:3]:
         q = O.spec_q(N, W)
         L = O.limbs_of(W)
